@@ -1,0 +1,891 @@
+// The C-ABI of libMMFFmulti_lib.so: every symbol that pyBall/MMFF_multi.py
+// binds (cpp/libs_OCL/MMFFmulti_lib.cpp, cpp/common/molecular/libMMFF.h,
+// cpp/common/libUtils.h), with the same C signatures, plus additive entry
+// points for uneven (CSR-packed) batches.  Declarations: include/mmff_multi.h.
+//
+// Error convention: the reference prints and calls exit(0); this library
+// prints to stderr and returns a failure value (NULL, -1 or false) instead.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <atomic>
+#include <string>
+#include <sys/stat.h>
+#include <thread>
+
+#include "world.h"
+
+extern "C" hipError_t fc_sample_angles(int kind, float k, float ang0, float r1, float r2, int n, const float* ang,
+                                       float* E, float* F);
+extern "C" hipError_t fc_sample_grid(const fcd::GridDev* g, int n, const float4* pos, float4 plq, float4* out);
+extern "C" hipError_t fc_sample_pair(int n, const float* r, float4 REQij, float2* out);
+
+using fcw::World;
+using fcw::world;
+
+namespace {
+
+void unsupported(const char* name) {
+    std::fprintf(stderr, "[libMMFFmulti_lib] %s: not part of the multi-system relaxation path; no-op\n", name);
+}
+
+bool file_exists(const std::string& f) {
+    struct stat st;
+    return stat(f.c_str(), &st) == 0;
+}
+void mkdirs(const std::string& d) {
+    std::string acc;
+    for (size_t i = 0; i < d.size(); i++) {
+        acc += d[i];
+        if (d[i] == '/' || i + 1 == d.size()) mkdir(acc.c_str(), 0755);
+    }
+}
+std::string basename_noext(const std::string& p) {
+    std::string b = p;
+    size_t k = b.find_last_of('/');
+    if (k != std::string::npos) b = b.substr(k + 1);
+    if (b.size() > 4 && b.substr(b.size() - 4) == ".xyz") b = b.substr(0, b.size() - 4);
+    return b;
+}
+
+// Surface + GridFF set-up (MolWorld_sp3::loadSurf, MolWorld_sp3.h:1058-1084, and
+// MolWorld_sp3_multi::initGridFF, MolWorld_sp3_multi.h:2501-2600): the B-spline
+// coefficients are read from data/<surface>/Bspline_PLQd.npy when present and
+// computed and saved there otherwise (GridFF::tryLoad_new, GridFF.h:1787-1866).
+bool setup_surface(const char* surf_name, double gridStep, fc::GridFFHost& g) {
+    std::string sn = surf_name;
+    fc::Surface S;
+    std::string base;
+    if (sn.rfind("synth:NaCl", 0) == 0) {
+        fc::make_nacl_slab(5.64, 2, 0.9, S);
+        base = "NaCl_synth";
+    } else {
+        std::string fx = sn;
+        if (fx.size() < 4 || fx.substr(fx.size() - 4) != ".xyz") fx += ".xyz";
+        if (!fc::load_surface_xyz(fx, S)) {
+            std::fprintf(stderr, "[libMMFFmulti_lib] cannot read surface %s\n", fx.c_str());
+            return false;
+        }
+        base = basename_noext(sn);
+    }
+    double ztop = -1e300;
+    for (int i = 0; i < S.natoms(); i++) ztop = std::max(ztop, S.apos[i * 3 + 2]);
+    const double ax = S.lvec[0], by = S.lvec[4], cz = S.lvec[8];
+    const std::string dir = "data/" + base;
+    const std::string fn = dir + "/Bspline_PLQd.npy";
+    std::vector<int> shape;
+    std::vector<double> data;
+    if (file_exists(fn) && fc::npy_load_f64(fn, shape, data) && shape.size() == 4 && shape[3] == 3) {
+        g.nx = shape[0];
+        g.ny = shape[1];
+        g.nz = shape[2];
+        g.step[0] = ax / g.nx;
+        g.step[1] = by / g.ny;
+        g.step[2] = cz / g.nz;
+        g.pos0[0] = -0.5 * ax;
+        g.pos0[1] = -0.5 * by;
+        g.pos0[2] = ztop;
+        g.alphaMorse = 1.5;
+        g.coefs = std::move(data);
+        return true;
+    }
+    const int nx = (int)std::lround(ax / gridStep), ny = (int)std::lround(by / gridStep),
+              nz = (int)std::lround(cz / gridStep);
+    fc::make_gridff_bspline(S, nx, ny, nz, cz / nz, ztop, 1.5, 1.0, 4, 16, g);
+    mkdirs(dir);
+    std::vector<int> sh = {g.nx, g.ny, g.nz, 3};
+    if (!fc::npy_save_f64(fn, sh, g.coefs.data()))
+        std::fprintf(stderr, "[libMMFFmulti_lib] warning: cannot write %s\n", fn.c_str());
+    return true;
+}
+
+bool finish_init(World& W) {
+    if (!W.ensure_device()) return false;
+    if (!W.upload_all(true, true, true)) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// =============================================================== libUtils.h
+void printBuffNames() {
+    World& W = world();
+    std::printf("buffers:  ");
+    for (auto& kv : W.buffers) std::printf("%s ", kv.first.c_str());
+    std::printf("\nfbuffers: ");
+    for (auto& kv : W.fbuffers) std::printf("%s ", kv.first.c_str());
+    std::printf("\nibuffers: ");
+    for (auto& kv : W.ibuffers) std::printf("%s ", kv.first.c_str());
+    std::printf("\n");
+}
+double* getBuff(const char* name) {
+    World& W = world();
+    auto it = W.buffers.find(name);
+    if (it == W.buffers.end()) {
+        std::printf("ERROR: getBuff(%s) NOT FOUND\n", name);
+        printBuffNames();
+        return nullptr;
+    }
+    return it->second;
+}
+float* getfBuff(const char* name) {
+    World& W = world();
+    auto it = W.fbuffers.find(name);
+    if (it == W.fbuffers.end()) {
+        std::printf("ERROR: getfBuff(%s) NOT FOUND\n", name);
+        printBuffNames();
+        return nullptr;
+    }
+    return it->second;
+}
+int* getIBuff(const char* name) {
+    World& W = world();
+    auto it = W.ibuffers.find(name);
+    if (it == W.ibuffers.end()) {
+        std::printf("ERROR: getIBuff(%s) NOT FOUND\n", name);
+        printBuffNames();
+        return nullptr;
+    }
+    return it->second;
+}
+
+// ========================================================= MMFFmulti_lib.cpp
+void init_buffers() { world().register_buffers(); }
+
+// xyz_name: "synth:natoms=<N>,seed=<s>,sigma=<A>" builds the template molecule
+// with the synthetic generator and nSys replicas, replica i perturbed with seed
+// s+1+i (the builder path from a typed .xyz is not part of this library).
+void* init(int nSys, char* xyz_name, char* surf_name, char* smile_name, bool bMMFF, bool bEpairs, int* nPBC,
+           int* grid_nPBC, double gridStep, char* sElementTypes, char* sAtomTypes, char* sBondTypes,
+           char* sAngleTypes, double T, double gamma, int nExplore, int nRelax, double pos_kick, double vel_kick,
+           int GridFF) {
+    (void)smile_name; (void)bEpairs; (void)grid_nPBC; (void)sElementTypes; (void)sAtomTypes; (void)sBondTypes;
+    (void)sAngleTypes; (void)T; (void)gamma; (void)nExplore; (void)nRelax; (void)pos_kick; (void)vel_kick;
+    World& W = world();
+    if (nPBC && (nPBC[0] || nPBC[1] || nPBC[2]))
+        std::fprintf(stderr, "[libMMFFmulti_lib] periodic molecules (nPBC>0) are not supported; using nPBC=0\n");
+    if (!xyz_name || std::strncmp(xyz_name, "synth:", 6) != 0) {
+        std::fprintf(stderr,
+                     "[libMMFFmulti_lib] init(%s): typed-topology building from .xyz is not part of this library; "
+                     "use xyz_name='synth:natoms=N,seed=S,sigma=X' or init_systems()/init_workload()\n",
+                     xyz_name ? xyz_name : "(null)");
+        return nullptr;
+    }
+    int natoms = 24;
+    long long seed = 0;
+    double sigma = 0.1;
+    {
+        std::string spec(xyz_name + 6);
+        size_t p = 0;
+        while (p < spec.size()) {
+            size_t q = spec.find(',', p);
+            std::string kv = spec.substr(p, q == std::string::npos ? std::string::npos : q - p);
+            size_t e = kv.find('=');
+            if (e != std::string::npos) {
+                std::string k = kv.substr(0, e), v = kv.substr(e + 1);
+                if (k == "natoms") natoms = std::atoi(v.c_str());
+                else if (k == "seed") seed = std::atoll(v.c_str());
+                else if (k == "sigma") sigma = std::atof(v.c_str());
+            }
+            if (q == std::string::npos) break;
+            p = q + 1;
+        }
+    }
+    fc::MolSys tmpl;
+    if (!fc::gen_molecule(natoms, (uint64_t)seed, tmpl)) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] init: cannot generate a %d-atom molecule\n", natoms);
+        return nullptr;
+    }
+    std::vector<fc::MolSys> ms(nSys, tmpl);
+    for (int i = 0; i < nSys; i++) fc::perturb_positions(ms[i], (uint64_t)(seed + 1 + i), sigma);
+    W.bMMFF = bMMFF;
+    W.bGridFF = false;
+    if (!W.set_systems(std::move(ms))) return nullptr;
+    if (surf_name && GridFF > 0) {
+        fc::GridFFHost g;
+        if (!setup_surface(surf_name, gridStep > 0 ? gridStep : 0.1, g)) return nullptr;
+        W.set_grid(g);
+    }
+    if (!finish_init(W)) return nullptr;
+    return &W;
+}
+
+int run(int nstepMax, double dt, double Fconv, int ialg, double* outE, double* outF, int iParalel) {
+    (void)dt;
+    (void)ialg;
+    World& W = world();
+    if (!W.dev_ready || W.nSys == 0) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] run: not initialised\n");
+        return -1;
+    }
+    int n = (iParalel == 2) ? W.relax_split(nstepMax, Fconv, false) : W.relax_persistent(nstepMax, Fconv, false);
+    if (n < 0) return n;
+    // state stays on the device for the next call: refresh the start copy
+    hipMemcpyAsync(W.d_atoms0.p, W.d_atoms.p, sizeof(float4) * W.ntotV, hipMemcpyDeviceToDevice, W.stream);
+    W.download_all(true, true);
+    if (outE)
+        for (int s = 0; s < W.nSys; s++) outE[s] = W.Esys[s];
+    if (outF)
+        for (int s = 0; s < W.nSys; s++) outF[s] = W.Fmax[s];
+    return n;
+}
+
+void MDloop(int perframe, double Ftol, int iParalel, int perVF) {
+    (void)perVF;
+    run(perframe, -1, Ftol > 0 ? Ftol : 1e-3, 0, nullptr, nullptr, iParalel);
+}
+
+void set_opt(double dt_max, double dt_min, double damp_max, double finc, double fdec, double falpha, int minLastNeg,
+             double cvf_min, double cvf_max) {
+    (void)cvf_min;
+    (void)cvf_max;
+    World& W = world();
+    W.fp.dt_max = (float)dt_max;
+    W.fp.dt_min = (float)dt_min;
+    W.fp.damp_max = (float)damp_max;
+    W.fp.finc = (float)finc;
+    W.fp.fdec = (float)fdec;
+    W.fp.falpha = (float)falpha;
+    W.fp.minLastNeg = minLastNeg;
+}
+
+void sampleSurf(char* name, int n, double* rs, double* Es, double* fs, int kind, int atyp, double Q, double K,
+                double RQ, double* pos0_, bool bSave);
+void sampleSurf_vecs(char* name, int n, double* poss_, double* Es, double* fs_, int kind, int atyp, double Q,
+                     double K, double RQ, double* pos0_, bool bSave);
+
+void pack_system(int isys, bool bParams, bool bForces, bool bVel, bool blvec) {
+    World& W = world();
+    if (isys < 0 || isys >= W.nSys) return;
+    W.pack_system(isys, bParams, bForces, bVel, blvec);
+}
+void unpack_system(int isys, bool bForces, bool bVel) {
+    World& W = world();
+    if (isys < 0 || isys >= W.nSys) return;
+    W.unpack_system(isys, bForces, bVel);
+}
+void upload_sys(int isys, bool bParams, bool bForces, bool bVel, bool blvec) {
+    (void)isys;
+    (void)blvec;
+    World& W = world();
+    W.upload_all(bParams, bForces, bVel);
+}
+void download_sys(int isys, bool bForces, bool bVel) {
+    (void)isys;
+    world().download_all(bForces, bVel);
+}
+void upload(bool bParams, bool bForces, bool bVel, bool blvec) {
+    (void)blvec;
+    World& W = world();
+    if (bParams) W.rebuild_exclusions();
+    W.upload_all(bParams, bForces, bVel);
+}
+void download(bool bForces, bool bVel) { world().download_all(bForces, bVel); }
+
+void change_lvec(double* lvec, bool bAdd, bool bUpdatePi) {
+    (void)lvec; (void)bAdd; (void)bUpdatePi;
+    unsupported("change_lvec");
+}
+void optimizeLattice_1d(double* dlvec, int n1, int n2, int initMode, double tol) {
+    (void)dlvec; (void)n1; (void)n2; (void)initMode; (void)tol;
+    unsupported("optimizeLattice_1d");
+}
+void optimizeLattice_2d_multi(double* dlvec, int nstesp, int initMode, double tol) {
+    (void)dlvec; (void)nstesp; (void)initMode; (void)tol;
+    unsupported("optimizeLattice_2d_multi");
+}
+void upload_pop(const char* fname) {
+    (void)fname;
+    unsupported("upload_pop");
+}
+void scan(int nconf, double* poss, double* rots, double* Es, double* aforces, double* aposs, bool omp, bool bRelax,
+          int niter_max, double dt, double Fconv, double Flim) {
+    (void)nconf; (void)poss; (void)rots; (void)Es; (void)aforces; (void)aposs; (void)omp; (void)bRelax;
+    (void)niter_max; (void)dt; (void)Fconv; (void)Flim;
+    unsupported("scan");
+}
+void setSwitches_multi(int CheckInvariants, int PBC, int NonBonded, int MMFF, int Angles, int PiSigma, int PiPiI,
+                       int dovdW, int bSaveToDatabase) {
+    World& W = world();
+    auto setb = [](bool& b, int i) { if (i > 0) b = true; else if (i < 0) b = false; };
+    auto seti = [](int& b, int i) { if (i > 0) b = 1; else if (i < 0) b = 0; };
+    setb(W.bCheckInvariants, CheckInvariants);
+    setb(W.bPBC, PBC);
+    setb(W.bNonBonded, NonBonded);
+    setb(W.bMMFF, MMFF);
+    seti(W.sw.doAngles, Angles);
+    seti(W.sw.doPiSigma, PiSigma);
+    seti(W.sw.doPiPiI, PiPiI);
+    setb(W.dovdW, dovdW);
+    setb(W.bSaveToDatabase, bSaveToDatabase);
+    if (W.nSys > 0) {
+        // bSubtractAngleNonBond follows bNonBonded (MMFFmulti_lib.cpp:328)
+        W.rebuild_exclusions();
+        if (W.dev_ready) {
+            W.d_excl.upload(W.excl, W.stream);
+            W.d_corr.upload(W.corr, W.stream);
+            W.d_info.upload(W.info, W.stream);
+            W.sync();
+        }
+    }
+}
+
+// ================================================================ libMMFF.h
+void setVerbosity(int verbosity_, int idebug_) {
+    (void)idebug_;
+    world().verbosity = verbosity_;
+    setbuf(stdout, NULL);
+}
+void insertSMILES(char* s) {
+    (void)s;
+    unsupported("insertSMILES");
+}
+bool checkInvariants(double maxVcog, double maxFcog, double maxTg) {
+    // net velocity, net force and torque of the template system's atoms
+    World& W = world();
+    const fc::MolSys& m = W.ffl;
+    double vc[3] = {0, 0, 0}, fc_[3] = {0, 0, 0}, tq[3] = {0, 0, 0};
+    for (int i = 0; i < m.natoms; i++) {
+        const double* p = &W.DOFs[i * 3];
+        const double* f = &W.fDOFs[i * 3];
+        const double* v = &W.vDOFs[i * 3];
+        for (int k = 0; k < 3; k++) {
+            vc[k] += v[k];
+            fc_[k] += f[k];
+        }
+        tq[0] += p[1] * f[2] - p[2] * f[1];
+        tq[1] += p[2] * f[0] - p[0] * f[2];
+        tq[2] += p[0] * f[1] - p[1] * f[0];
+    }
+    auto n = [](const double* a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); };
+    return (n(vc) <= maxVcog) && (n(fc_) <= maxFcog) && (n(tq) <= maxTg);
+}
+void setOptLog(int n, double* cos, double* f, double* v, double* dt, double* damp) {
+    (void)n; (void)cos; (void)f; (void)v; (void)dt; (void)damp;
+    unsupported("setOptLog");
+}
+bool setTrjName(const char* trj_fname_, int savePerNsteps_, int* nPBC) {
+    (void)trj_fname_; (void)savePerNsteps_; (void)nPBC;
+    unsupported("setTrjName");
+    return false;
+}
+int saveXYZ(const char* fname, const char* comment, int imod) {
+    (void)imod;
+    World& W = world();
+    FILE* f = std::fopen(fname, "w");
+    if (!f) return -1;
+    static const char* el[4] = {"C", "N", "O", "H"};
+    for (int s = 0; s < W.nSys; s++) {
+        const fcd::SysInfo& I = W.info[s];
+        std::fprintf(f, "%d\n%s sys=%d\n", I.natoms, comment ? comment : "", s);
+        for (int i = 0; i < I.natoms; i++) {
+            const float4 a = W.atoms[I.v0 + i];
+            std::fprintf(f, "%s %.6f %.6f %.6f\n", el[W.elem[I.a0 + i] & 3], a.x, a.y, a.z);
+        }
+    }
+    std::fclose(f);
+    return W.nSys;
+}
+const char* getTypeName(int ia, bool fromFF) {
+    (void)fromFF;
+    static const char* el[4] = {"C", "N", "O", "H"};
+    World& W = world();
+    if (ia < 0 || ia >= W.ffl.natoms || W.ffl.elem.empty()) return "?";
+    return el[W.ffl.elem[ia] & 3];
+}
+double eval() {
+    World& W = world();
+    if (!W.dev_ready || W.nSys == 0) return NAN;
+    if (!W.eval_forces()) return NAN;
+    W.download_all(true, false);
+    return W.Esys[0];
+}
+void addDistConstrain(int i0, int i1, double lmin, double lmax, double kmin, double kmax, double flim, double* shift,
+                      bool bOldIndex) {
+    (void)i0; (void)i1; (void)lmin; (void)lmax; (void)kmin; (void)kmax; (void)flim; (void)shift; (void)bOldIndex;
+    unsupported("addDistConstrain");
+}
+void addAngConstrain(int i0, int i1, int i2, double ang0, double k) {
+    (void)i0; (void)i1; (void)i2; (void)ang0; (void)k;
+    unsupported("addAngConstrain");
+}
+static void apply_to_template(int n, const int* sel, void (*fn)(double*, const double*), const double* arg) {
+    World& W = world();
+    if (W.nSys == 0) return;
+    fc::MolSys& m = W.mols[0];
+    if (!sel || n <= 0) {
+        for (int i = 0; i < m.natoms; i++) fn(&m.apos[i * 3], arg);
+    } else {
+        for (int k = 0; k < n; k++)
+            if (sel[k] >= 0 && sel[k] < m.natoms) fn(&m.apos[sel[k] * 3], arg);
+    }
+    W.pack_system(0, false, false, false, false);
+    W.ffl.apos = m.apos;
+    W.DOFs = m.apos;
+    W.register_buffers();
+    if (W.dev_ready) W.upload_all(false, false, false);
+}
+void shift_atoms_ax(int n, int* selection, double* d) {
+    apply_to_template(n, selection, [](double* p, const double* a) {
+        p[0] += a[0];
+        p[1] += a[1];
+        p[2] += a[2];
+    }, d);
+}
+void shift_atoms(int n, int* selection, int ia0, int ia1, double l) {
+    World& W = world();
+    if (W.nSys == 0) return;
+    const fc::MolSys& m = W.mols[0];
+    if (ia0 < 0 || ia1 < 0 || ia0 >= m.natoms || ia1 >= m.natoms) return;
+    double d[3];
+    double nn = 0;
+    for (int k = 0; k < 3; k++) {
+        d[k] = m.apos[ia1 * 3 + k] - m.apos[ia0 * 3 + k];
+        nn += d[k] * d[k];
+    }
+    nn = std::sqrt(nn);
+    for (int k = 0; k < 3; k++) d[k] *= l / nn;
+    shift_atoms_ax(n, selection, d);
+}
+void rotate_atoms_ax(int n, int* selection, double* p0, double* ax, double phi) {
+    double arg[7] = {p0 ? p0[0] : 0, p0 ? p0[1] : 0, p0 ? p0[2] : 0, ax ? ax[0] : 0, ax ? ax[1] : 0,
+                     ax ? ax[2] : 1, phi};
+    apply_to_template(n, selection, [](double* p, const double* a) {
+        double u[3] = {a[3], a[4], a[5]};
+        double nu = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        for (double& x : u) x /= nu;
+        double v[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+        double c = std::cos(a[6]), s = std::sin(a[6]);
+        double uv = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+        double cx[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        for (int k = 0; k < 3; k++) p[k] = a[k] + v[k] * c + cx[k] * s + u[k] * uv * (1 - c);
+    }, arg);
+}
+void rotate_atoms(int n, int* selection, int ia0, int iax0, int iax1, double phi) {
+    World& W = world();
+    if (W.nSys == 0) return;
+    const fc::MolSys& m = W.mols[0];
+    if (ia0 < 0 || iax0 < 0 || iax1 < 0 || ia0 >= m.natoms || iax0 >= m.natoms || iax1 >= m.natoms) return;
+    double p0[3], ax[3];
+    for (int k = 0; k < 3; k++) {
+        p0[k] = m.apos[ia0 * 3 + k];
+        ax[k] = m.apos[iax1 * 3 + k] - m.apos[iax0 * 3 + k];
+    }
+    rotate_atoms_ax(n, selection, p0, ax, phi);
+}
+void scanTranslation_ax(int n, int* selection, double* vec, int nstep, double* Es, const char* trjName,
+                        bool bAddjustCaps) {
+    (void)n; (void)selection; (void)vec; (void)nstep; (void)Es; (void)trjName; (void)bAddjustCaps;
+    unsupported("scanTranslation_ax");
+}
+void scanTranslation(int n, int* selection, int ia0, int ia1, double l, int nstep, double* Es, const char* trjName,
+                     bool bAddjustCaps) {
+    (void)n; (void)selection; (void)ia0; (void)ia1; (void)l; (void)nstep; (void)Es; (void)trjName;
+    (void)bAddjustCaps;
+    unsupported("scanTranslation");
+}
+void scanRotation_ax(int n, int* selection, double* p0, double* ax, double phi, int nstep, double* Es,
+                     const char* trjName) {
+    (void)n; (void)selection; (void)p0; (void)ax; (void)phi; (void)nstep; (void)Es; (void)trjName;
+    unsupported("scanRotation_ax");
+}
+void scanRotation(int n, int* selection, int ia0, int iax0, int iax1, double phi, int nstep, double* Es,
+                  const char* trjName) {
+    (void)n; (void)selection; (void)ia0; (void)iax0; (void)iax1; (void)phi; (void)nstep; (void)Es; (void)trjName;
+    unsupported("scanRotation");
+}
+
+// ---- term samplers: evaluated by the device's FP32 terms (libMMFF.h:426-497)
+static void sample_angles_dev(int kind, double k, double ang0, double r1, double r2, int n, const double* angles,
+                              double* Es, double* Fs) {
+    World& W = world();
+    if (n <= 0 || !W.ensure_device()) return;
+    std::vector<float> a(n), e(n), f(n);
+    for (int i = 0; i < n; i++) a[i] = (float)angles[i];
+    float *da, *de, *df;
+    hipMalloc(&da, n * 4);
+    hipMalloc(&de, n * 4);
+    hipMalloc(&df, n * 4);
+    hipMemcpy(da, a.data(), n * 4, hipMemcpyHostToDevice);
+    fcw::hip_ok(fc_sample_angles(kind, (float)k, (float)ang0, (float)r1, (float)r2, n, da, de, df), "sample_angles");
+    hipMemcpy(e.data(), de, n * 4, hipMemcpyDeviceToHost);
+    hipMemcpy(f.data(), df, n * 4, hipMemcpyDeviceToHost);
+    hipFree(da);
+    hipFree(de);
+    hipFree(df);
+    for (int i = 0; i < n; i++) {
+        Es[i] = e[i];
+        Fs[i] = f[i];
+    }
+}
+void sample_DistConstr(double lmin, double lmax, double kmin, double kmax, double flim, int n, double* xs, double* Es,
+                       double* Fs) {
+    (void)lmin; (void)lmax; (void)kmin; (void)kmax; (void)flim; (void)xs;
+    unsupported("sample_DistConstr");
+    for (int i = 0; i < n; i++) Es[i] = Fs[i] = 0;
+}
+void sample_evalPiAling(double k, double ang0, double r1, double r2, int n, double* angles, double* Es, double* Fs) {
+    sample_angles_dev(0, k, ang0, r1, r2, n, angles, Es, Fs);
+}
+void sample_evalAngleCos(double k, double ang0, double r1, double r2, int n, double* angles, double* Es, double* Fs) {
+    sample_angles_dev(1, k, ang0, r1, r2, n, angles, Es, Fs);
+}
+void sample_evalAngleCosHalf(double k, double ang0, double r1, double r2, int n, double* angles, double* Es,
+                             double* Fs) {
+    sample_angles_dev(2, k, ang0, r1, r2, n, angles, Es, Fs);
+}
+// kind 2 samples this library's pair term getLJQH (not the reference's
+// addAtomicForceLJQ); the other kinds are not part of the path.
+void sampleNonBond(int n, double* rs, double* Es, double* fs, int kind, double* REQi_, double* REQj_, double K,
+                   double Rdamp) {
+    (void)K;
+    (void)Rdamp;
+    World& W = world();
+    if (n <= 0) return;
+    if (kind != 2 || !W.ensure_device()) {
+        unsupported("sampleNonBond(kind!=2)");
+        for (int i = 0; i < n; i++) Es[i] = fs[i] = 0;
+        return;
+    }
+    float4 req = make_float4((float)(REQi_[0] + REQj_[0]), (float)(REQi_[1] * REQj_[1]), (float)(REQi_[2] * REQj_[2]),
+                             (float)(REQi_[3] * REQj_[3]));
+    std::vector<float> r(n);
+    std::vector<float2> o(n);
+    for (int i = 0; i < n; i++) r[i] = (float)rs[i];
+    float* dr;
+    float2* dout;
+    hipMalloc(&dr, n * 4);
+    hipMalloc(&dout, n * 8);
+    hipMemcpy(dr, r.data(), n * 4, hipMemcpyHostToDevice);
+    fcw::hip_ok(fc_sample_pair(n, dr, req, dout), "sampleNonBond");
+    hipMemcpy(o.data(), dout, n * 8, hipMemcpyDeviceToHost);
+    hipFree(dr);
+    hipFree(dout);
+    for (int i = 0; i < n; i++) {
+        fs[i] = o[i].x;
+        Es[i] = o[i].y;
+    }
+}
+
+// GridFF probe sampling (MMFFmulti_lib.cpp:168-250): kinds >= 10 sample this
+// library's B-spline grid with a probe of element `atyp` (0=C,1=N,2=O,3=H) and
+// charge Q; other kinds are not part of the path.
+static void sample_grid_dev(int n, const double* p3, double* Es, double* f3, int ncomp, int atyp, double Q,
+                            double K) {
+    World& W = world();
+    if (n <= 0 || !W.ensure_device() || W.grid_f4.empty()) {
+        for (int i = 0; i < n; i++) Es[i] = 0;
+        return;
+    }
+    if (!W.d_grid.p || W.d_grid.n != W.grid_f4.size()) {
+        W.d_grid.upload(W.grid_f4, W.stream);
+        W.sync();
+    }
+    const double R0[4] = {1.9255, 1.8300, 1.7500, 1.4430};
+    const double E0[4] = {0.00455323095, 0.00299212319, 0.00260184625, 0.0020};
+    int t = (atyp >= 0 && atyp < 4) ? atyp : 0;
+    double alpha = K > 0 ? K : W.grid.alphaMorse;
+    double e = std::exp(alpha * R0[t]);
+    double cL = e * std::sqrt(E0[t]);
+    float4 plq = make_float4((float)(e * cL), (float)cL, (float)Q, 0.f);
+    std::vector<float4> pos(n), out(n);
+    for (int i = 0; i < n; i++) pos[i] = make_float4((float)p3[i * 3], (float)p3[i * 3 + 1], (float)p3[i * 3 + 2], 0);
+    float4 *dp, *dout;
+    hipMalloc(&dp, n * 16);
+    hipMalloc(&dout, n * 16);
+    hipMemcpy(dp, pos.data(), n * 16, hipMemcpyHostToDevice);
+    fcd::RunParams P = W.run_params(0, 0);
+    P.grid.coefs = W.d_grid.p;
+    fcw::hip_ok(fc_sample_grid(&P.grid, n, dp, plq, dout), "sampleSurf");
+    hipMemcpy(out.data(), dout, n * 16, hipMemcpyDeviceToHost);
+    hipFree(dp);
+    hipFree(dout);
+    for (int i = 0; i < n; i++) {
+        Es[i] = out[i].w;
+        if (ncomp == 1) f3[i] = out[i].z;
+        else {
+            f3[i * 3] = out[i].x;
+            f3[i * 3 + 1] = out[i].y;
+            f3[i * 3 + 2] = out[i].z;
+        }
+    }
+}
+void sampleSurf(char* name, int n, double* rs, double* Es, double* fs, int kind, int atyp, double Q, double K,
+                double RQ, double* pos0_, bool bSave) {
+    (void)RQ;
+    (void)bSave;
+    World& W = world();
+    if (name) {
+        fc::GridFFHost g;
+        if (setup_surface(name, 0.1, g)) W.set_grid(g);
+    }
+    if (kind < 10) {
+        unsupported("sampleSurf(kind<10)");
+        for (int i = 0; i < n; i++) Es[i] = fs[i] = 0;
+        return;
+    }
+    std::vector<double> p((size_t)n * 3);
+    for (int i = 0; i < n; i++) {
+        p[i * 3] = pos0_ ? pos0_[0] : 0;
+        p[i * 3 + 1] = pos0_ ? pos0_[1] : 0;
+        p[i * 3 + 2] = rs[i];
+    }
+    sample_grid_dev(n, p.data(), Es, fs, 1, atyp, Q, K);
+}
+void sampleSurf_vecs(char* name, int n, double* poss_, double* Es, double* fs_, int kind, int atyp, double Q,
+                     double K, double RQ, double* pos0_, bool bSave) {
+    (void)RQ;
+    (void)bSave;
+    (void)pos0_;
+    World& W = world();
+    if (name) {
+        fc::GridFFHost g;
+        if (setup_surface(name, 0.1, g)) W.set_grid(g);
+    }
+    if (kind < 10) {
+        unsupported("sampleSurf_vecs(kind<10)");
+        for (int i = 0; i < n; i++) Es[i] = 0;
+        return;
+    }
+    sample_grid_dev(n, poss_, Es, fs_, 3, atyp, Q, K);
+}
+
+// ================================================================ additive
+// Build a batch from CSR-packed host arrays (one entry per system in natoms /
+// nnode; per-atom/per-node rows concatenated in system order).
+int init_systems(int nSys, const int* natoms, const int* nnode, const double* apos, const double* REQs,
+                 const int* neighs, const double* apars, const double* bLs, const double* bKs, const double* Ksp,
+                 const double* Kpp, const int* elem) {
+    World& W = world();
+    std::vector<fc::MolSys> ms(nSys);
+    size_t ia = 0, in = 0, iv = 0;
+    for (int s = 0; s < nSys; s++) {
+        fc::MolSys& m = ms[s];
+        m.natoms = natoms[s];
+        m.nnode = nnode[s];
+        const int N = m.natoms, M = m.nnode, V = N + M;
+        if (N <= 0 || M < 0 || M > N) return -1;
+        m.apos.assign(apos + iv * 3, apos + (iv + V) * 3);
+        m.REQs.assign(REQs + ia * 4, REQs + (ia + N) * 4);
+        m.neighs.assign(neighs + ia * 4, neighs + (ia + N) * 4);
+        for (int x : m.neighs)
+            if (x >= N) return -1;
+        m.apars.assign(apars + in * 4, apars + (in + M) * 4);
+        m.bLs.assign(bLs + in * 4, bLs + (in + M) * 4);
+        m.bKs.assign(bKs + in * 4, bKs + (in + M) * 4);
+        m.Ksp.assign(Ksp + in * 4, Ksp + (in + M) * 4);
+        m.Kpp.assign(Kpp + in * 4, Kpp + (in + M) * 4);
+        if (elem) m.elem.assign(elem + ia, elem + ia + N);
+        else m.elem.assign(N, 0);
+        ia += N;
+        in += M;
+        iv += V;
+    }
+    if (!W.set_systems(std::move(ms))) return -1;
+    if (!finish_init(W)) return -1;
+    return W.ntotA;
+}
+
+// Synthetic workloads.
+//  kind 0: one molecule type of nmin atoms (seed seed0), nSys replicas each
+//          perturbed with seed seed0+1+i (uniform batch)
+//  kind 1: nSys independent molecules, natoms ~ U[nmin,nmax], seed seed0+i
+//  kind 2: nSys pairs of molecules, each ~ U[nmin,nmax] atoms, random relative
+//          orientation, 4-8 A apart
+// surf_name (may be NULL) places every system 3.0 +- 0.3 A above the substrate
+// and loads/creates its GridFF.
+int init_workload(int kind, int nSys, long long seed0, int nmin, int nmax, double sigma, const char* surf_name,
+                  double gridStep) {
+    World& W = world();
+    std::vector<fc::MolSys> ms(nSys);
+    fc::GridFFHost g;
+    bool surf = surf_name && surf_name[0];
+    if (surf && !setup_surface(surf_name, gridStep > 0 ? gridStep : 0.1, g)) return -1;
+    const double ax = g.nx * g.step[0], by = g.ny * g.step[1];
+    std::atomic<bool> fail{false};
+    if (kind == 0) {
+        fc::MolSys tmpl;
+        if (!fc::gen_molecule(nmin, (uint64_t)seed0, tmpl)) return -1;
+        for (int i = 0; i < nSys; i++) {
+            ms[i] = tmpl;
+            fc::perturb_positions(ms[i], (uint64_t)(seed0 + 1 + i), sigma);
+        }
+    } else {
+        const int nth = 16;
+        std::vector<std::thread> th;
+        for (int t = 0; t < nth; t++)
+            th.emplace_back([&, t]() {
+                for (int i = t; i < nSys; i += nth) {
+                    const uint64_t sd = (uint64_t)(seed0 + i);
+                    fc::Rng rng(sd * 2654435761ull + 17);
+                    fc::MolSys m;
+                    if (kind == 1) {
+                        int n = nmin + rng.randint(nmax - nmin + 1);
+                        if (!fc::gen_molecule(n, sd, m)) { fail = true; continue; }
+                    } else {
+                        fc::MolSys a, b;
+                        int na = nmin + rng.randint(nmax - nmin + 1);
+                        int nb = nmin + rng.randint(nmax - nmin + 1);
+                        if (!fc::gen_molecule(na, sd * 2 + 0x1000000ull, a) ||
+                            !fc::gen_molecule(nb, sd * 2 + 0x1000001ull, b)) { fail = true; continue; }
+                        fc::place_molecule(a, sd * 3 + 1, 0.0, 0.0, 0.0);
+                        fc::place_molecule(b, sd * 3 + 2, 0.0, 0.0, 0.0);
+                        // separate the centroids by 4-8 A along a random direction,
+                        // pushed apart until the closest pair is >= 2.5 A
+                        double dir[3] = {rng.normal(), rng.normal(), rng.normal()};
+                        double nd = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+                        for (double& x : dir) x /= nd;
+                        double sep = rng.uniform(4.0, 8.0);
+                        auto centroid = [](const fc::MolSys& x, double* c) {
+                            c[0] = c[1] = c[2] = 0;
+                            for (int k = 0; k < x.natoms; k++)
+                                for (int q = 0; q < 3; q++) c[q] += x.apos[k * 3 + q] / x.natoms;
+                        };
+                        double ca[3], cb[3];
+                        centroid(a, ca);
+                        centroid(b, cb);
+                        for (int it = 0; it < 200; it++) {
+                            double shift[3];
+                            for (int q = 0; q < 3; q++) shift[q] = ca[q] + dir[q] * sep - cb[q];
+                            double dmin = 1e9;
+                            for (int x = 0; x < a.natoms; x++)
+                                for (int y = 0; y < b.natoms; y++) {
+                                    double d2 = 0;
+                                    for (int q = 0; q < 3; q++) {
+                                        double d = a.apos[x * 3 + q] - (b.apos[y * 3 + q] + shift[q]);
+                                        d2 += d * d;
+                                    }
+                                    dmin = std::min(dmin, d2);
+                                }
+                            if (std::sqrt(dmin) >= 2.5) {
+                                for (int y = 0; y < b.natoms; y++)
+                                    for (int q = 0; q < 3; q++) b.apos[y * 3 + q] += shift[q];
+                                break;
+                            }
+                            sep += 0.25;
+                        }
+                        fc::merge_molecules(a, b, m);
+                    }
+                    fc::perturb_positions(m, sd + 7777777ull, sigma);
+                    if (surf) {
+                        fc::place_molecule(m, sd + 31337ull, g.pos0[2] + 3.0 + rng.uniform(-0.3, 0.3),
+                                           g.pos0[0] + rng.uniform(0, ax), g.pos0[1] + rng.uniform(0, by));
+                    }
+                    ms[i] = std::move(m);
+                }
+            });
+        for (auto& x : th) x.join();
+    }
+    if (fail) return -1;
+    W.bGridFF = false;
+    if (!W.set_systems(std::move(ms))) return -1;
+    if (surf) W.set_grid(g);
+    if (!finish_init(W)) return -1;
+    return W.ntotA;
+}
+
+// Relax the whole batch on the device.
+//   path 0: persistent kernel; 1: split-phase kernels
+//   restore_start: restart from the positions of the last init/upload
+//   bDownload: copy atoms/forces/velocities back into the host mirrors
+// Returns the largest number of FIRE steps taken by any system, -1 on error.
+int relax(int nstepMax, double Fconv, int path, bool restore_start, bool bDownload) {
+    World& W = world();
+    if (!W.dev_ready || W.nSys == 0) return -1;
+    int n = (path == 1) ? W.relax_split(nstepMax, Fconv, restore_start) : W.relax_persistent(nstepMax, Fconv, restore_start);
+    if (n >= 0 && bDownload) W.download_all(true, true);
+    return n;
+}
+
+double get_last_ms() { return world().last_ms; }
+
+// Evaluate forces and energies of every system at the current positions
+// (split kernels, no move); results in gpu_aforces / sys_E after download.
+int eval_forces() {
+    World& W = world();
+    if (!W.eval_forces()) return -1;
+    return W.download_all(true, false) ? 0 : -1;
+}
+
+void set_conv_mode(int mode) { world().conv_mode = mode; }
+
+// nSys, ntotA, ntotM, ntotV, maxN, maxM, grid nx, ny, nz
+void get_batch_sizes(int* out) {
+    World& W = world();
+    out[0] = W.nSys;
+    out[1] = W.ntotA;
+    out[2] = W.ntotM;
+    out[3] = W.ntotV;
+    out[4] = W.maxN;
+    out[5] = W.maxM;
+    out[6] = W.grid.nx;
+    out[7] = W.grid.ny;
+    out[8] = W.grid.nz;
+}
+
+// Grid geometry: pos0[3], step[3], alphaMorse; returns the coefficient count
+int get_grid_info(double* out7) {
+    World& W = world();
+    for (int k = 0; k < 3; k++) {
+        out7[k] = W.grid.pos0[k];
+        out7[3 + k] = W.grid.step[k];
+    }
+    out7[6] = W.grid.alphaMorse;
+    return (int)W.grid.coefs.size();
+}
+double* get_grid_coefs() { return world().grid.coefs.data(); }
+
+// Host-only helpers (no device): generator and topology, for inspection.
+int gen_molecule_arrays(int natoms, long long seed, double sigma, long long pseed, int* nnode_out, double* apos,
+                        double* REQs, int* neighs, double* apars, double* bLs, double* bKs, double* Ksp, double* Kpp,
+                        int* elem) {
+    fc::MolSys m;
+    if (!fc::gen_molecule(natoms, (uint64_t)seed, m)) return -1;
+    if (sigma > 0) fc::perturb_positions(m, (uint64_t)pseed, sigma);
+    *nnode_out = m.nnode;
+    if (apos) std::memcpy(apos, m.apos.data(), m.apos.size() * 8);
+    if (REQs) std::memcpy(REQs, m.REQs.data(), m.REQs.size() * 8);
+    if (neighs) std::memcpy(neighs, m.neighs.data(), m.neighs.size() * 4);
+    if (apars) std::memcpy(apars, m.apars.data(), m.apars.size() * 8);
+    if (bLs) std::memcpy(bLs, m.bLs.data(), m.bLs.size() * 8);
+    if (bKs) std::memcpy(bKs, m.bKs.data(), m.bKs.size() * 8);
+    if (Ksp) std::memcpy(Ksp, m.Ksp.data(), m.Ksp.size() * 8);
+    if (Kpp) std::memcpy(Kpp, m.Kpp.data(), m.Kpp.size() * 8);
+    if (elem) std::memcpy(elem, m.elem.data(), m.elem.size() * 4);
+    return m.nnode;
+}
+int make_back_neighs(int natoms, int nnode, const int* neighs, int* bkneighs) {
+    return fc::make_back_neighs(natoms, nnode, neighs, bkneighs) ? 0 : -1;
+}
+int make_exclusions(int natoms, int nnode, const int* neighs, int subtract13, unsigned* mask_out, int* W_out,
+                    int* ncorr_out) {
+    std::vector<uint32_t> mask;
+    std::vector<fc::PairCorr> corr;
+    int W = 0;
+    fc::make_exclusions(natoms, nnode, neighs, subtract13 != 0, mask, W, corr);
+    if (mask_out) std::memcpy(mask_out, mask.data(), mask.size() * 4);
+    *W_out = W;
+    *ncorr_out = (int)corr.size();
+    return 0;
+}
+// NaCl slab grid (direct sums + B-spline fit) without touching the device.
+int make_nacl_gridff(int nx, int ny, int nz, double dz, int nthreads, double* coefs_out, double* geom_out7) {
+    fc::Surface S;
+    fc::make_nacl_slab(5.64, 2, 0.9, S);
+    fc::GridFFHost g;
+    fc::make_gridff_bspline(S, nx, ny, nz, dz, 0.0, 1.5, 1.0, 4, nthreads, g);
+    if (coefs_out) std::memcpy(coefs_out, g.coefs.data(), g.coefs.size() * 8);
+    for (int k = 0; k < 3; k++) {
+        geom_out7[k] = g.pos0[k];
+        geom_out7[3 + k] = g.step[k];
+    }
+    geom_out7[6] = g.alphaMorse;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// Host-side data structures of the multi-system MMFF relaxation library.
+//
+// One relaxation "system" is one molecule (nodes first, caps after, then one
+// pi-vector per node), stored in the layout of MMFFsp3_loc
+// (cpp/common/molecular/MMFFsp3_loc.h:40-130 and MMFFBuilder.h:4153-4282):
+//   vecs   [0,nnode)            node atoms (<=4 bonded neighbours + pi-vector)
+//          [nnode,natoms)       cap atoms (one neighbour)
+//          [natoms,natoms+nnode) pi-vectors of the nodes
+//   REQs   (R0, sqrt(E0), Q, H) per atom
+//   neighs 4 local atom indices per atom, -1 padded (caps: {node,-1,-1,-1})
+//   apars  (cos(Ass/2), sin(Ass/2), 4*Kss, sin(Asp)) per node
+//   bLs, bKs, Ksp, Kpp: 4 per node (one per neighbour slot)
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fc {
+
+enum Elem { EL_C = 0, EL_N = 1, EL_O = 2, EL_H = 3 };
+
+struct MolSys {
+    int natoms = 0;
+    int nnode = 0;
+    std::vector<double> apos;   // (natoms+nnode)*3
+    std::vector<double> REQs;   // natoms*4
+    std::vector<int> neighs;    // natoms*4
+    std::vector<double> apars;  // nnode*4
+    std::vector<double> bLs;    // nnode*4
+    std::vector<double> bKs;    // nnode*4
+    std::vector<double> Ksp;    // nnode*4
+    std::vector<double> Kpp;    // nnode*4
+    std::vector<int> elem;      // natoms (EL_*)
+    std::vector<double> fapos;  // (natoms+nnode)*3, filled by unpack with forces
+    std::vector<double> vapos;  // (natoms+nnode)*3, filled by unpack with velocities
+    int nvecs() const { return natoms + nnode; }
+};
+
+// ---------------- deterministic RNG (splitmix64 + Box-Muller) --------------
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }  // [0,1)
+    double uniform(double a, double b) { return a + (b - a) * uniform(); }
+    int randint(int n) { return (int)(uniform() * n); }
+    double normal();
+    bool has_spare = false;
+    double spare = 0;
+};
+
+// ---------------- generator (molgen.cpp) ----------------
+// Random branched tree of C/N/O sp3/sp2 centres with H caps, exactly `natoms`
+// atoms, ideal geometry (no noise).  Returns false when natoms is not
+// reachable (natoms < 3).
+bool gen_molecule(int natoms, uint64_t seed, MolSys& out);
+// Adds N(0, sigma) noise to every atom position (pi-vectors untouched).
+void perturb_positions(MolSys& m, uint64_t seed, double sigma);
+// Random rigid rotation, then translation so that the lowest atom sits at
+// z = ztarget and the centroid at (cx, cy).
+void place_molecule(MolSys& m, uint64_t seed, double ztarget, double cx, double cy);
+// Flip pi-vectors into the half-space of `ax` (MMFFsp3_loc.h:1167-1172).
+void flip_pis(MolSys& m, const double ax[3]);
+// Pair of molecules in one system (config 4): concatenated topology.
+void merge_molecules(const MolSys& a, const MolSys& b, MolSys& out);
+
+// ---------------- topology (topology.cpp) ----------------
+// makeBackNeighs (MMFFsp3_loc.h:1335-1358): bk[j] lists ia*4+k for every node
+// ia with neighs[ia][k]==j, filled first-empty.  Returns false on >4.
+bool make_back_neighs(int natoms, int nnode, const int* neighs, int* bkneighs);
+// Non-bonded pair weights in the bNonBondNeighs strategy:
+//   w(i,j) = [j not bonded to i] - (#nodes k whose angle loop subtracts {i,j})
+// Pairs with w==1 need nothing; every other pair (including i==j) is marked in
+// the exclusion bit-matrix (row i, bit j).  Pairs with w not in {0,1} are also
+// returned in `corr` as (i,j,w) so that w*pair(i,j) can be added back.
+struct PairCorr { int i, j; float w; };
+void make_exclusions(int natoms, int nnode, const int* neighs, bool subtract13,
+                     std::vector<uint32_t>& mask, int& words_per_row,
+                     std::vector<PairCorr>& corr);
+
+// ---------------- GridFF (gridff_host.cpp) ----------------
+struct GridFFHost {
+    int nx = 0, ny = 0, nz = 0;
+    double pos0[3] = {0, 0, 0};
+    double step[3] = {0.1, 0.1, 0.1};   // orthogonal grid spacing
+    double alphaMorse = 1.5;
+    std::vector<double> coefs;          // (nx,ny,nz,3) C-order, z fastest
+};
+bool npy_load_f64(const std::string& fname, std::vector<int>& shape, std::vector<double>& data);
+bool npy_save_f64(const std::string& fname, const std::vector<int>& shape, const double* data);
+// Surface atoms: positions (n*3) and REQs (n*4) with a 2D lattice (a, b).
+struct Surface {
+    std::vector<double> apos, REQs;
+    double lvec[9] = {0};
+    int natoms() const { return (int)(apos.size() / 3); }
+};
+void make_nacl_slab(double a, int nlayers, double q, Surface& s);
+bool load_surface_xyz(const std::string& fname, Surface& s);
+bool save_surface_xyz(const std::string& fname, const Surface& s);
+// Direct lattice sums of the Pauli / London / damped-Coulomb channels on an
+// (nx,ny,nz) grid over one surface cell, then exact cubic B-spline
+// interpolation coefficients (periodic in x,y).
+void make_gridff_bspline(const Surface& s, int nx, int ny, int nz, double dz,
+                         double z0, double alphaMorse, double Rdamp, int nimg_coul,
+                         int nthreads, GridFFHost& g);
+
+}  // namespace fc

@@ -1,0 +1,68 @@
+// Kernel-side descriptors shared by the host world and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ff_device.h"
+
+namespace fcd {
+
+// One relaxation system in the CSR-packed batch.
+struct SysInfo {
+    int natoms, nnode;
+    int a0;     // first atom   (REQs, neighs, PLQs, excl rows)
+    int n0;     // first node   (apars, bLs, bKs, Ksp, Kpp; fneigh block n0*8)
+    int v0;     // first vec    (atoms, avel, aforce, bkNeighs)
+    int mask0;  // first word of the exclusion bit-matrix (natoms rows of W words)
+    int corr0, ncorr;  // weighted pair corrections
+};
+
+struct PairCorrDev {
+    int i, j;
+    float w, pad;
+};
+
+// Status codes written per system.
+enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXSTEP = 2, ST_NAN = 3 };
+
+// Convergence criterion.
+enum { CONV_FMAX = 0, CONV_F2SUM = 1 };
+
+struct BatchDev {
+    int nsys;
+    const SysInfo* info;
+    float4* atoms;      // [sum V] positions (atoms, then pi-vectors)
+    float4* avel;       // [sum V]
+    float4* aforce;     // [sum V]
+    const float4* REQs; // [sum N] (R0, sqrt(E0), Q, H)
+    const float4* PLQs; // [sum N] (cP, cL, Q, cH)
+    const int4* neighs; // [sum N] local indices
+    const int4* bkNeighs;  // [sum V] reference convention: absolute fneigh index
+    const float4* apars;   // [sum M]
+    const float4* bLs;
+    const float4* bKs;
+    const float4* Ksp;
+    const float4* Kpp;
+    const unsigned* excl;  // exclusion bit-matrix rows
+    const PairCorrDev* corr;
+    float4* fneigh;        // [8 * sum M] (split path only)
+    float* ebond;          // [sum M] (split path only)
+    // per-system results
+    int* nsteps;
+    int* status;
+    double* Esys;
+    float* Fmax;
+    float4* fire;          // per-system FIRE state (dt, damping, lastNeg, -) for the split path
+};
+
+struct RunParams {
+    FFSwitch sw;
+    int bNonBonded;
+    int bGridFF;
+    GridDev grid;
+    FireParams fire;
+    float Fconv2;
+    int conv_mode;
+    int nstep_max;
+};
+
+}  // namespace fcd

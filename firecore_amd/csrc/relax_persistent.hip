@@ -1,0 +1,362 @@
+// Persistent batched FIRE relaxation on gfx950.
+//
+// One workgroup relaxes one system at a time, entirely on the device: it
+// pulls a system index from a global work queue (systems pre-sorted by
+// decreasing cost, so the largest start first), stages the system in LDS,
+// iterates force evaluation + per-system FIRE until the system's own Fmax is
+// below tolerance (or it hits the step limit / goes non-finite), writes the
+// result back and pulls the next system.  Retirement is therefore per system
+// with no host round trip and no grid-wide barrier.
+//
+// Per FIRE step (reference recipe MolWorld_sp3::run_no_omp,
+// MolWorld_sp3.h:2291-2474):
+//   A. forces: thread i -> non-bonded sum of atom i over all j of its system
+//      (positions/params broadcast from LDS, exclusions from an LDS bit
+//      matrix) + B-spline GridFF sample; thread k -> bonded terms of node k
+//      (recoils into LDS, MMFFsp3_loc.h:250-575)
+//   B. assemble: thread v -> own force + recoils via back-neighbours, pi force
+//      projected perpendicular to the pi-vector (assemble_atom, :998-1021);
+//      block reduction of <v|f>, <v|v>, <f|f> and max|f|^2
+//   C. per-system FIRE (DynamicOpt.cpp:250-267) and move (move_atom_FIRE,
+//      MMFFsp3_loc.h:1264-1281)
+#include "kernels.h"
+
+namespace fcd {
+
+template <int BLOCK>
+struct BlockRed {
+    static constexpr int NW = BLOCK / 64;
+};
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// LDS layout (bytes), computed identically on the host (world.cpp).
+struct LdsLayout {
+    int nmax, mmax, wmax;
+    __host__ __device__ int vmax() const { return nmax + mmax; }
+    __host__ __device__ int off_pos() const { return 0; }
+    __host__ __device__ int off_req() const { return off_pos() + vmax() * 16; }
+    __host__ __device__ int off_fnb() const { return off_req() + nmax * 16; }
+    __host__ __device__ int off_fself() const { return off_fnb() + nmax * 16; }
+    __host__ __device__ int off_fng() const { return off_fself() + 2 * mmax * 16; }
+    __host__ __device__ int off_mask() const { return off_fng() + 8 * mmax * 16; }
+    __host__ __device__ int bytes() const { return off_mask() + nmax * wmax * 4; }
+};
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, RunParams P, const int* order,
+                                                                 int nqueue, int* queue, LdsLayout L) {
+    extern __shared__ float4 lds_dyn[];
+    char* lds = reinterpret_cast<char*>(lds_dyn);
+    float4* s_pos = reinterpret_cast<float4*>(lds + L.off_pos());
+    float4* s_req = reinterpret_cast<float4*>(lds + L.off_req());
+    float4* s_fnb = reinterpret_cast<float4*>(lds + L.off_fnb());
+    float4* s_fself = reinterpret_cast<float4*>(lds + L.off_fself());
+    float4* s_fng = reinterpret_cast<float4*>(lds + L.off_fng());
+    unsigned* s_mask = reinterpret_cast<unsigned*>(lds + L.off_mask());
+    constexpr int NW = BLOCK / 64;
+    __shared__ float s_red[4][NW];
+    __shared__ double s_redd[NW];
+    __shared__ int s_q;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wid = tid >> 6;
+
+    for (;;) {
+        if (tid == 0) s_q = atomicAdd(queue, 1);
+        __syncthreads();
+        const int q = s_q;
+        __syncthreads();
+        if (q >= nqueue) break;
+        const int s = order[q];
+        const SysInfo I = B.info[s];
+        const int N = I.natoms, M = I.nnode, V = N + M;
+        const int W = (N + 31) >> 5;
+
+        // ---- stage the system in LDS
+        for (int v = tid; v < V; v += BLOCK) s_pos[v] = B.atoms[I.v0 + v];
+        for (int i = tid; i < N; i += BLOCK) s_req[i] = B.REQs[I.a0 + i];
+        // transposed to [word][atom] so that lanes read consecutive banks
+        for (int w = tid; w < N * W; w += BLOCK) s_mask[(w % W) * N + (w / W)] = B.excl[I.mask0 + w];
+
+        // ---- per-thread persistent state: atom tid, node tid, vecs tid and tid+BLOCK
+        const bool hasAtom = tid < N;
+        const bool hasNode = tid < M;
+        float4 plq = hasAtom ? B.PLQs[I.a0 + tid] : make_float4(0, 0, 0, 0);
+        int4 ng = make_int4(-1, -1, -1, -1);
+        float4 apar = make_float4(0, 0, 0, 0), bL = apar, bK = apar, ksp = apar, kpp = apar;
+        if (hasNode) {
+            ng = B.neighs[I.a0 + tid];
+            apar = B.apars[I.n0 + tid];
+            bL = B.bLs[I.n0 + tid];
+            bK = B.bKs[I.n0 + tid];
+            ksp = B.Ksp[I.n0 + tid];
+            kpp = B.Kpp[I.n0 + tid];
+        }
+        int4 bk[2];
+        V3 vel[2];
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int v = tid + r * BLOCK;
+            bk[r] = make_int4(-1, -1, -1, -1);
+            vel[r] = zero3();
+            if (v < V) {
+                // back-neighbours as local indices into s_fng (sigma rows [0,4M), pi rows [4M,8M))
+                int4 b = B.bkNeighs[I.v0 + v];
+                const int base = I.n0 * 8;
+                bk[r].x = b.x >= 0 ? b.x - base : -1;
+                bk[r].y = b.y >= 0 ? b.y - base : -1;
+                bk[r].z = b.z >= 0 ? b.z - base : -1;
+                bk[r].w = b.w >= 0 ? b.w - base : -1;
+            }
+        }
+        FireState fs;
+        fs.dt = P.fire.dt_max;
+        fs.damping = P.fire.damp_max;
+        fs.lastNeg = 0;
+        __syncthreads();
+
+        int step = 0;
+        int status = ST_RUNNING;
+        float fmax2_last = 0.f;
+        V3 fv[2];
+        for (;;) {
+            // ================= A: forces
+            float epart = 0.f;
+            if (hasAtom) {
+                const int i = tid;
+                const float4 pi4 = s_pos[i];
+                const V3 pi = xyz(pi4);
+                const float4 ri = s_req[i];
+                const float kQi = COULOMB_K * ri.z;
+                V3 f = zero3();
+                float e = 0.f;
+                if (P.bNonBonded) {
+                    for (int w = 0; w < W; w++) {
+                        const unsigned m = s_mask[w * N + i];
+                        const int j0 = w << 5;
+                        const int jn = min(32, N - j0);
+                        for (int jj = 0; jj < jn; jj++) {
+                            const int j = j0 + jj;
+                            const float4 pj = s_pos[j];
+                            const float4 rj = s_req[j];
+                            const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                            float fr, ep;
+                            lj_pair(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                            const bool ex = (m >> jj) & 1u;
+                            fr = ex ? 0.f : fr;
+                            ep = ex ? 0.f : ep;
+                            f.x -= d.x * fr;
+                            f.y -= d.y * fr;
+                            f.z -= d.z * fr;
+                            e += ep;
+                        }
+                    }
+                    // weighted corrections (ring topologies; empty for trees)
+                    for (int c = 0; c < I.ncorr; c++) {
+                        const PairCorrDev pc = B.corr[I.corr0 + c];
+                        if (pc.i != i) continue;
+                        const float4 pj = s_pos[pc.j];
+                        const float4 rj = s_req[pc.j];
+                        const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                        float fr, ep;
+                        lj_pair(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                        f -= d * (fr * pc.w);
+                        e += ep * pc.w;
+                    }
+                    e *= 0.5f;
+                }
+                if (P.bGridFF) e += grid_sample(P.grid, pi, mk(plq.x, plq.y, plq.z), f);
+                s_fnb[i] = make_float4(f.x, f.y, f.z, 0.f);
+                epart += e;
+            }
+            if (hasNode) {
+                V3 fbs[4], fps[4], fa, fpi;
+                auto pos = [&](int v) { return xyz(s_pos[v]); };
+                epart += eval_node(tid, N, M, pos, ng, apar, bL, bK, ksp, kpp, P.sw, fbs, fps, fa, fpi);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    s_fng[tid * 4 + k] = make_float4(fbs[k].x, fbs[k].y, fbs[k].z, 0.f);
+                    s_fng[4 * M + tid * 4 + k] = make_float4(fps[k].x, fps[k].y, fps[k].z, 0.f);
+                }
+                s_fself[tid] = make_float4(fa.x, fa.y, fa.z, 0.f);
+                s_fself[M + tid] = make_float4(fpi.x, fpi.y, fpi.z, 0.f);
+            }
+            __syncthreads();
+
+            // ================= B: assemble + reductions
+            float vf = 0.f, vv = 0.f, ff = 0.f, fm2 = 0.f;
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const int v = tid + r * BLOCK;
+                fv[r] = zero3();
+                if (v < V) {
+                    V3 f = zero3();
+                    const bool isPi = v >= N;
+                    if (!isPi) {
+                        f = xyz(s_fnb[v]);
+                        if (v < M) f += xyz(s_fself[v]);
+                    } else {
+                        f = xyz(s_fself[M + (v - N)]);
+                    }
+                    const int b4[4] = {bk[r].x, bk[r].y, bk[r].z, bk[r].w};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        if (b4[k] < 0) break;
+                        f += xyz(s_fng[b4[k]]);
+                    }
+                    if (isPi) {
+                        const V3 p = xyz(s_pos[v]);
+                        f -= p * dot(p, f);
+                    }
+                    fv[r] = f;
+                    vf += dot(vel[r], f);
+                    vv += dot(vel[r], vel[r]);
+                    const float f2 = dot(f, f);
+                    ff += f2;
+                    fm2 = fmaxf(fm2, f2);
+                }
+            }
+            vf = wave_sum(vf);
+            vv = wave_sum(vv);
+            ff = wave_sum(ff);
+            fm2 = wave_max(fm2);
+            if (lane == 0) {
+                s_red[0][wid] = vf;
+                s_red[1][wid] = vv;
+                s_red[2][wid] = ff;
+                s_red[3][wid] = fm2;
+            }
+            __syncthreads();
+            vf = s_red[0][0];
+            vv = s_red[1][0];
+            ff = s_red[2][0];
+            fm2 = s_red[3][0];
+#pragma unroll
+            for (int w = 1; w < NW; w++) {
+                vf += s_red[0][w];
+                vv += s_red[1][w];
+                ff += s_red[2][w];
+                fm2 = fmaxf(fm2, s_red[3][w]);
+            }
+            fmax2_last = fm2;
+            const float crit = (P.conv_mode == CONV_F2SUM) ? ff : fm2;
+            if (!(ff == ff) || isinf(ff)) status = ST_NAN;
+            else if (crit < P.Fconv2) status = ST_CONVERGED;
+            else if (step >= P.nstep_max) status = ST_MAXSTEP;
+            if (status != ST_RUNNING) {
+                // FP64 reduction of the energy at this geometry
+                double ed = wave_sum_d((double)epart);
+                if (lane == 0) s_redd[wid] = ed;
+                __syncthreads();
+                if (tid == 0) {
+                    double E = 0.0;
+                    for (int w = 0; w < NW; w++) E += s_redd[w];
+                    B.Esys[s] = E;
+                    B.nsteps[s] = step;
+                    B.status[s] = status;
+                    B.Fmax[s] = sqrtf(fm2);
+                }
+                break;
+            }
+
+            // ================= C: FIRE + move
+            float cv, cfe;
+            fire_update(P.fire, fs, vf, vv, ff, cv, cfe);
+            const float dt = fs.dt;
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const int v = tid + r * BLOCK;
+                if (v < V) {
+                    const bool isPi = v >= N;
+                    V3 p = xyz(s_pos[v]);
+                    V3 f = fv[r];
+                    if (isPi) f -= p * dot(p, f);
+                    V3 w = vel[r] * cv + f * (dt + cfe);
+                    if (isPi) w -= p * dot(p, w);
+                    p += w * dt;
+                    if (isPi) p = p * (1.f / sqrtf(dot(p, p)));
+                    vel[r] = w;
+                    s_pos[v] = make_float4(p.x, p.y, p.z, 0.f);
+                }
+            }
+            step++;
+            __syncthreads();
+        }
+        (void)fmax2_last;
+        // ---- write back the final state of this system
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int v = tid + r * BLOCK;
+            if (v < V) {
+                B.atoms[I.v0 + v] = s_pos[v];
+                B.avel[I.v0 + v] = make_float4(vel[r].x, vel[r].y, vel[r].z, 0.f);
+                B.aforce[I.v0 + v] = make_float4(fv[r].x, fv[r].y, fv[r].z, 0.f);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template __global__ void relax_persistent_kernel<64>(BatchDev, RunParams, const int*, int, int*, LdsLayout);
+template __global__ void relax_persistent_kernel<128>(BatchDev, RunParams, const int*, int, int*, LdsLayout);
+template __global__ void relax_persistent_kernel<256>(BatchDev, RunParams, const int*, int, int*, LdsLayout);
+
+}  // namespace fcd
+
+// Host launcher (C linkage inside the library).
+extern "C" hipError_t fc_launch_relax_persistent(int block, int ngroups, size_t lds_bytes, hipStream_t stream,
+                                                 const fcd::BatchDev* B, const fcd::RunParams* P, const int* order,
+                                                 int nqueue, int* queue, int nmax, int mmax, int wmax) {
+    fcd::LdsLayout L{nmax, mmax, wmax};
+    switch (block) {
+        case 64:
+            hipLaunchKernelGGL(fcd::relax_persistent_kernel<64>, dim3(ngroups), dim3(64), lds_bytes, stream, *B, *P,
+                               order, nqueue, queue, L);
+            break;
+        case 128:
+            hipLaunchKernelGGL(fcd::relax_persistent_kernel<128>, dim3(ngroups), dim3(128), lds_bytes, stream, *B,
+                               *P, order, nqueue, queue, L);
+            break;
+        case 256:
+            hipLaunchKernelGGL(fcd::relax_persistent_kernel<256>, dim3(ngroups), dim3(256), lds_bytes, stream, *B,
+                               *P, order, nqueue, queue, L);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax) {
+    fcd::LdsLayout L{nmax, mmax, wmax};
+    return L.bytes();
+}
+
+extern "C" hipError_t fc_persistent_occupancy(int block, size_t lds, int* nblocks) {
+    switch (block) {
+        case 64:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(nblocks, fcd::relax_persistent_kernel<64>, 64, lds);
+        case 128:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(nblocks, fcd::relax_persistent_kernel<128>, 128, lds);
+        case 256:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(nblocks, fcd::relax_persistent_kernel<256>, 256, lds);
+        default:
+            return hipErrorInvalidValue;
+    }
+}

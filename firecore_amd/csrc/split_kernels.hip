@@ -1,0 +1,267 @@
+// Split-phase kernels: one launch per phase of a FIRE step, over all live
+// systems of the batch, in the reference's packed buffer layout
+// (OCL_MM.h:197-250 / MolWorld_sp3_multi.h:514-592).  They mirror the
+// reference's kernel decomposition (getNonBond_GridFF_Bspline, getMMFFf4,
+// updateAtomsMMFFf4 in relax_multi.cl) but carry the CPU path's physics and a
+// full per-system FIRE controller on the device.  They are the parity
+// reference for the persistent kernel and the HBM-bound form of the path.
+#include "kernels.h"
+
+namespace fcd {
+
+// ---------------------------------------------------------------------------
+// Non-bonded (+GridFF) force per atom.  grid = (ceil(maxN/64), nsys), 64 threads.
+// aforce[v0+i] = (f, 0.5*E_nb + E_grid)   (assignment, like the reference NB kernel)
+__global__ void __launch_bounds__(64) k_nonbond_grid(BatchDev B, RunParams P) {
+    __shared__ float4 s_p[64];
+    __shared__ float4 s_r[64];
+    const int s = blockIdx.y;
+    const SysInfo I = B.info[s];
+    if (B.status[s] != ST_RUNNING) return;
+    const int N = I.natoms;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (blockIdx.x * 64 >= N) return;
+    const bool act = i < N;
+    const int W = (N + 31) >> 5;
+    V3 pi = zero3();
+    float4 ri = make_float4(0, 0, 0, 0);
+    if (act) {
+        pi = xyz(B.atoms[I.v0 + i]);
+        ri = B.REQs[I.a0 + i];
+    }
+    const float kQi = COULOMB_K * ri.z;
+    V3 f = zero3();
+    float e = 0.f;
+    if (P.bNonBonded) {
+        for (int j0 = 0; j0 < N; j0 += 64) {
+            const int jl = j0 + threadIdx.x;
+            __syncthreads();
+            if (jl < N) {
+                s_p[threadIdx.x] = B.atoms[I.v0 + jl];
+                s_r[threadIdx.x] = B.REQs[I.a0 + jl];
+            }
+            __syncthreads();
+            const int jn = min(64, N - j0);
+            if (act) {
+                unsigned m0 = B.excl[I.mask0 + i * W + (j0 >> 5)];
+                unsigned m1 = ((j0 >> 5) + 1 < W) ? B.excl[I.mask0 + i * W + (j0 >> 5) + 1] : 0u;
+                for (int jj = 0; jj < jn; jj++) {
+                    const float4 pj = s_p[jj];
+                    const float4 rj = s_r[jj];
+                    const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                    float fr, ep;
+                    lj_pair(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                    const unsigned m = jj < 32 ? m0 : m1;
+                    const bool ex = (m >> (jj & 31)) & 1u;
+                    fr = ex ? 0.f : fr;
+                    ep = ex ? 0.f : ep;
+                    f.x -= d.x * fr;
+                    f.y -= d.y * fr;
+                    f.z -= d.z * fr;
+                    e += ep;
+                }
+            }
+        }
+        if (act) {
+            for (int c = 0; c < I.ncorr; c++) {
+                const PairCorrDev pc = B.corr[I.corr0 + c];
+                if (pc.i != i) continue;
+                const V3 pj = xyz(B.atoms[I.v0 + pc.j]);
+                const float4 rj = B.REQs[I.a0 + pc.j];
+                const V3 d = pj - pi;
+                float fr, ep;
+                lj_pair(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                f -= d * (fr * pc.w);
+                e += ep * pc.w;
+            }
+        }
+        e *= 0.5f;
+    }
+    if (!act) return;
+    if (P.bGridFF) {
+        const float4 plq = B.PLQs[I.a0 + i];
+        e += grid_sample(P.grid, pi, mk(plq.x, plq.y, plq.z), f);
+    }
+    B.aforce[I.v0 + i] = make_float4(f.x, f.y, f.z, e);
+}
+
+// ---------------------------------------------------------------------------
+// Bonded terms per node.  One thread per node of the batch; node_sys maps the
+// global node index to its system.  Writes recoils into fneigh (sigma block
+// then pi block of each system, reference layout), adds the node's own force
+// to aforce[node] and assigns aforce[pi] (as getMMFFf4 does).
+__global__ void __launch_bounds__(256) k_bonded(BatchDev B, RunParams P, const int* node_sys, int ntot_nodes) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ntot_nodes) return;
+    const int s = node_sys[g];
+    if (B.status[s] != ST_RUNNING) return;
+    const SysInfo I = B.info[s];
+    const int k = g - I.n0;
+    const int N = I.natoms, M = I.nnode;
+    const float4* apos = B.atoms + I.v0;
+    auto pos = [&](int v) { return xyz(apos[v]); };
+    V3 fbs[4], fps[4], fa, fpi;
+    float E = eval_node(k, N, M, pos, B.neighs[I.a0 + k], B.apars[g], B.bLs[g], B.bKs[g], B.Ksp[g], B.Kpp[g], P.sw,
+                        fbs, fps, fa, fpi);
+    float4* fng = B.fneigh + (size_t)I.n0 * 8;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        fng[k * 4 + q] = make_float4(fbs[q].x, fbs[q].y, fbs[q].z, 0.f);
+        fng[4 * M + k * 4 + q] = make_float4(fps[q].x, fps[q].y, fps[q].z, 0.f);
+    }
+    float4 a = B.aforce[I.v0 + k];
+    a.x += fa.x;
+    a.y += fa.y;
+    a.z += fa.z;
+    B.aforce[I.v0 + k] = a;
+    B.aforce[I.v0 + N + k] = make_float4(fpi.x, fpi.y, fpi.z, 0.f);
+    B.ebond[g] = E;
+}
+
+__device__ __forceinline__ float blk_sum(float v, float* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    float r = 0.f;
+    for (int w = 0; w < nw; w++) r += sh[w];
+    return r;
+}
+__device__ __forceinline__ float blk_max(float v, float* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    float r = sh[0];
+    for (int w = 1; w < nw; w++) r = fmaxf(r, sh[w]);
+    return r;
+}
+__device__ __forceinline__ double blk_sum_d(double v, double* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int w = 0; w < nw; w++) r += sh[w];
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Assemble + per-system reduction + FIRE + move.  One block (256) per system.
+// do_move=0 only assembles (force/energy evaluation for parity checks).
+__global__ void __launch_bounds__(256) k_update(BatchDev B, RunParams P, int do_move, int step) {
+    __shared__ float sh[8];
+    __shared__ double shd[8];
+    const int s = blockIdx.x;
+    if (B.status[s] != ST_RUNNING) return;
+    const SysInfo I = B.info[s];
+    const int N = I.natoms, M = I.nnode, V = N + M;
+    const float4* fng = B.fneigh;  // bkNeighs are absolute indices into fneigh
+    float vf = 0.f, vv = 0.f, ff = 0.f, fm2 = 0.f;
+    double e = 0.0;
+    for (int v = threadIdx.x; v < V; v += blockDim.x) {
+        const float4 a = B.aforce[I.v0 + v];
+        V3 f = xyz(a);
+        if (v < N) e += (double)a.w;
+        const int4 b = B.bkNeighs[I.v0 + v];
+        if (b.x >= 0) { f += xyz(fng[b.x]);
+        if (b.y >= 0) { f += xyz(fng[b.y]);
+        if (b.z >= 0) { f += xyz(fng[b.z]);
+        if (b.w >= 0) { f += xyz(fng[b.w]); } } } }
+        if (v >= N) {
+            const V3 p = xyz(B.atoms[I.v0 + v]);
+            f -= p * dot(p, f);
+        }
+        const V3 w = xyz(B.avel[I.v0 + v]);
+        vf += dot(w, f);
+        vv += dot(w, w);
+        const float f2 = dot(f, f);
+        ff += f2;
+        fm2 = fmaxf(fm2, f2);
+        B.aforce[I.v0 + v] = make_float4(f.x, f.y, f.z, 0.f);
+    }
+    for (int k = threadIdx.x; k < M; k += blockDim.x) e += (double)B.ebond[I.n0 + k];
+    vf = blk_sum(vf, sh);
+    vv = blk_sum(vv, sh);
+    ff = blk_sum(ff, sh);
+    fm2 = blk_max(fm2, sh);
+    e = blk_sum_d(e, shd);
+    if (threadIdx.x == 0) {
+        B.Esys[s] = e;
+        B.Fmax[s] = sqrtf(fm2);
+        B.nsteps[s] = step;
+    }
+    if (!do_move) return;
+    int status = ST_RUNNING;
+    const float crit = (P.conv_mode == CONV_F2SUM) ? ff : fm2;
+    if (!(ff == ff) || isinf(ff)) status = ST_NAN;
+    else if (crit < P.Fconv2) status = ST_CONVERGED;
+    else if (step >= P.nstep_max) status = ST_MAXSTEP;
+    if (status != ST_RUNNING) {
+        if (threadIdx.x == 0) B.status[s] = status;
+        return;
+    }
+    float4 st = B.fire[s];
+    FireState fs;
+    fs.dt = st.x;
+    fs.damping = st.y;
+    fs.lastNeg = (int)st.z;
+    float cv, cfe;
+    fire_update(P.fire, fs, vf, vv, ff, cv, cfe);
+    __syncthreads();
+    if (threadIdx.x == 0) B.fire[s] = make_float4(fs.dt, fs.damping, (float)fs.lastNeg, 0.f);
+    const float dt = fs.dt;
+    for (int v = threadIdx.x; v < V; v += blockDim.x) {
+        const bool isPi = v >= N;
+        V3 p = xyz(B.atoms[I.v0 + v]);
+        V3 f = xyz(B.aforce[I.v0 + v]);
+        if (isPi) f -= p * dot(p, f);
+        V3 w = xyz(B.avel[I.v0 + v]) * cv + f * (dt + cfe);
+        if (isPi) w -= p * dot(p, w);
+        p += w * dt;
+        if (isPi) p = p * (1.f / sqrtf(dot(p, p)));
+        B.avel[I.v0 + v] = make_float4(w.x, w.y, w.z, 0.f);
+        B.atoms[I.v0 + v] = make_float4(p.x, p.y, p.z, 0.f);
+    }
+}
+
+// Reset per-system state before a split-path relaxation.
+__global__ void k_reset_state(BatchDev B, RunParams P) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= B.nsys) return;
+    B.status[s] = ST_RUNNING;
+    B.nsteps[s] = 0;
+    B.fire[s] = make_float4(P.fire.dt_max, P.fire.damp_max, 0.f, 0.f);
+}
+__global__ void k_zero_f4(float4* a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = make_float4(0, 0, 0, 0);
+}
+
+}  // namespace fcd
+
+extern "C" hipError_t fc_launch_split_step(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P,
+                                           const int* node_sys, int ntot_nodes, int maxN, int do_move, int step) {
+    dim3 g1((maxN + 63) / 64, B->nsys);
+    hipLaunchKernelGGL(fcd::k_nonbond_grid, g1, dim3(64), 0, st, *B, *P);
+    hipLaunchKernelGGL(fcd::k_bonded, dim3((ntot_nodes + 255) / 256), dim3(256), 0, st, *B, *P, node_sys, ntot_nodes);
+    hipLaunchKernelGGL(fcd::k_update, dim3(B->nsys), dim3(256), 0, st, *B, *P, do_move, step);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t fc_launch_reset(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P) {
+    hipLaunchKernelGGL(fcd::k_reset_state, dim3((B->nsys + 255) / 256), dim3(256), 0, st, *B, *P);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t fc_launch_zero_f4(hipStream_t st, float4* a, int n) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fcd::k_zero_f4, dim3((n + 255) / 256), dim3(256), 0, st, a, n);
+    return hipGetLastError();
+}

@@ -1,0 +1,78 @@
+// Topology bookkeeping: back-neighbours and non-bonded exclusions.
+#include "fc_host.h"
+
+#include <cstring>
+
+namespace fc {
+
+// Back-neighbour lists, following MMFFsp3_loc::makeBackNeighs
+// (cpp/common/molecular/MMFFsp3_loc.h:1335-1358): for every node ia and slot
+// j with neighs[ia][j]=ja>=0, append ia*4+j to the first empty entry of
+// bkneighs[ja].  Cap atoms are not scanned (their own neighbour lists point
+// back to their node).
+bool make_back_neighs(int natoms, int nnode, const int* neighs, int* bk) {
+    for (int i = 0; i < natoms * 4; i++) bk[i] = -1;
+    for (int ia = 0; ia < nnode; ia++) {
+        for (int j = 0; j < 4; j++) {
+            int ja = neighs[ia * 4 + j];
+            if (ja < 0) continue;
+            if (ja >= natoms) return false;
+            int* row = bk + ja * 4;
+            int k = 0;
+            while (k < 4 && row[k] >= 0) k++;
+            if (k == 4) return false;
+            row[k] = ia * 4 + j;
+        }
+    }
+    return true;
+}
+
+// Pair weights of the non-bonded sum in the "bNonBondNeighs" strategy of the
+// reference CPU path:
+//  * NBFF::evalLJQs_ng4_atom_omp (NBFF.h:666-691) skips j==i and j in
+//    neighs[i] (1-2 exclusion by the neighbour list);
+//  * MMFFsp3_loc::eval_atom (MMFFsp3_loc.h:466-552) subtracts the full pair
+//    interaction once for every pair of neighbours of a node (1-3) when
+//    doAngles && bSubtractAngleNonBond.
+// The net weight of the directed pair i->j is therefore
+//    w = [j not in neighs[i]] - n13(i,j)
+// which is 0 for plain 1-2 and 1-3 pairs of a tree and 1 for all other pairs.
+void make_exclusions(int natoms, int nnode, const int* neighs, bool subtract13,
+                     std::vector<uint32_t>& mask, int& W, std::vector<PairCorr>& corr) {
+    W = (natoms + 31) / 32;
+    mask.assign((size_t)natoms * W, 0u);
+    corr.clear();
+    std::vector<int> w((size_t)natoms * natoms, 1);
+    for (int i = 0; i < natoms; i++) {
+        w[(size_t)i * natoms + i] = 0;
+        for (int k = 0; k < 4; k++) {
+            int j = neighs[i * 4 + k];
+            if (j >= 0 && j < natoms) w[(size_t)i * natoms + j] = 0;
+        }
+    }
+    if (subtract13) {
+        for (int k = 0; k < nnode; k++) {
+            const int* ng = neighs + k * 4;
+            for (int a = 0; a < 3; a++) {
+                int i = ng[a];
+                if (i < 0) break;
+                for (int b = a + 1; b < 4; b++) {
+                    int j = ng[b];
+                    if (j < 0) break;
+                    w[(size_t)i * natoms + j] -= 1;
+                    w[(size_t)j * natoms + i] -= 1;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < natoms; i++) {
+        for (int j = 0; j < natoms; j++) {
+            int wij = w[(size_t)i * natoms + j];
+            if (wij == 1) continue;
+            mask[(size_t)i * W + (j >> 5)] |= (1u << (j & 31));
+            if (i != j && wij != 0) corr.push_back({i, j, (float)wij});
+        }
+    }
+}
+
+}  // namespace fc

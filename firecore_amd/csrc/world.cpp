@@ -1,0 +1,587 @@
+// Host world: CSR packing of uneven batches, device buffers and the device
+// entry points (persistent relaxation, split-phase relaxation, force
+// evaluation).  Packing conventions follow MolWorld_sp3_multi::pack_system
+// (MolWorld_sp3_multi.h:514-592): neighs/neighCell are system-local,
+// bkNeighs are absolute indices into fneigh (+n0*8 for atoms, +n0*8+4*nnode
+// for pi rows, datatypes_utils.h:19 copy_add).
+#include "world.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+extern "C" hipError_t fc_launch_relax_persistent(int block, int ngroups, size_t lds_bytes, hipStream_t stream,
+                                                 const fcd::BatchDev* B, const fcd::RunParams* P, const int* order,
+                                                 int nqueue, int* queue, int nmax, int mmax, int wmax);
+extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax);
+extern "C" hipError_t fc_launch_split_step(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P,
+                                           const int* node_sys, int ntot_nodes, int maxN, int do_move, int step);
+extern "C" hipError_t fc_launch_reset(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P);
+extern "C" hipError_t fc_persistent_occupancy(int block, size_t lds, int* nblocks);
+
+namespace fcw {
+
+bool hip_ok(hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    std::fprintf(stderr, "[libMMFFmulti_lib] HIP error in %s: %s\n", what, hipGetErrorString(e));
+    return false;
+}
+
+template <class T>
+void DevArr<T>::alloc(size_t cnt) {
+    if (cnt == n && p) return;
+    free_();
+    n = cnt;
+    if (cnt == 0) return;
+    hip_ok(hipMalloc((void**)&p, cnt * sizeof(T)), "hipMalloc");
+}
+template <class T>
+void DevArr<T>::free_() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+}
+template <class T>
+void DevArr<T>::upload(const std::vector<T>& h, hipStream_t s) {
+    alloc(std::max<size_t>(h.size(), 1));
+    if (!h.empty()) hip_ok(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+template <class T>
+void DevArr<T>::download(std::vector<T>& h, hipStream_t s) const {
+    size_t cnt = std::min(h.size(), n);
+    if (cnt) hip_ok(hipMemcpyAsync(h.data(), p, cnt * sizeof(T), hipMemcpyDeviceToHost, s), "download");
+}
+
+World& world() {
+    static World W;
+    return W;
+}
+
+void World::clear() {
+    mols.clear();
+    info.clear();
+    nSys = ntotA = ntotM = ntotV = maxN = maxM = 0;
+}
+
+static inline float4 f4(double a, double b, double c, double d) {
+    return make_float4((float)a, (float)b, (float)c, (float)d);
+}
+
+bool World::set_systems(std::vector<fc::MolSys>&& ms) {
+    mols = std::move(ms);
+    nSys = (int)mols.size();
+    if (nSys == 0) return false;
+    ffl = mols[0];
+    build_packed();
+    register_buffers();
+    return true;
+}
+
+void World::rebuild_exclusions() {
+    excl.clear();
+    corr.clear();
+    const bool sub13 = bNonBonded && sw.doAngles;
+    for (int s = 0; s < nSys; s++) {
+        const fc::MolSys& m = mols[s];
+        std::vector<uint32_t> mask;
+        std::vector<fc::PairCorr> pc;
+        int W = 0;
+        fc::make_exclusions(m.natoms, m.nnode, m.neighs.data(), sub13, mask, W, pc);
+        info[s].mask0 = (int)excl.size();
+        info[s].corr0 = (int)corr.size();
+        info[s].ncorr = (int)pc.size();
+        excl.insert(excl.end(), mask.begin(), mask.end());
+        for (auto& c : pc) corr.push_back({c.i, c.j, c.w, 0.f});
+    }
+}
+
+void World::build_packed() {
+    info.assign(nSys, fcd::SysInfo{});
+    ntotA = ntotM = ntotV = maxN = maxM = 0;
+    bool uniform = true;
+    for (int s = 0; s < nSys; s++) {
+        const fc::MolSys& m = mols[s];
+        info[s].natoms = m.natoms;
+        info[s].nnode = m.nnode;
+        info[s].a0 = ntotA;
+        info[s].n0 = ntotM;
+        info[s].v0 = ntotV;
+        ntotA += m.natoms;
+        ntotM += m.nnode;
+        ntotV += m.natoms + m.nnode;
+        maxN = std::max(maxN, m.natoms);
+        maxM = std::max(maxM, m.nnode);
+        if (m.natoms != mols[0].natoms || m.nnode != mols[0].nnode) uniform = false;
+    }
+    natoms_uniform = uniform ? mols[0].natoms : 0;
+    nnode_uniform = uniform ? mols[0].nnode : 0;
+    nvecs_uniform = uniform ? mols[0].nvecs() : 0;
+    atoms.assign(ntotV, f4(0, 0, 0, 0));
+    aforces.assign(ntotV, f4(0, 0, 0, 0));
+    avel.assign(ntotV, f4(0, 0, 0, 0));
+    REQs.assign(ntotA, f4(0, 0, 0, 0));
+    PLQs.assign(ntotA, f4(0, 0, 0, 0));
+    constr.assign(ntotA, f4(-1, -1, -1, -1));
+    neighs.assign(ntotA, make_int4(-1, -1, -1, -1));
+    neighCell.assign(ntotA, make_int4(-1, -1, -1, -1));
+    bkNeighs.assign(ntotV, make_int4(-1, -1, -1, -1));
+    MMpars.assign(ntotM, f4(0, 0, 0, 0));
+    BLs = BKs = Ksp = Kpp = MMpars;
+    node_sys.assign(ntotM, 0);
+    elem.assign(ntotA, 0);
+    lvecs.assign((size_t)nSys * 3, f4(0, 0, 0, 0));
+    ilvecs.assign((size_t)nSys * 3, f4(0, 0, 0, 0));
+    pbcshifts.assign((size_t)nSys, f4(0, 0, 0, 0));
+    nsteps.assign(nSys, 0);
+    status.assign(nSys, 0);
+    Esys.assign(nSys, 0.0);
+    Fmax.assign(nSys, 0.f);
+    for (int s = 0; s < nSys; s++) pack_system(s, true, false, false, true);
+    rebuild_exclusions();
+    // template views
+    ndims[0] = ffl.nvecs() * 3;
+    ndims[1] = ffl.nnode;
+    ndims[2] = ffl.natoms - ffl.nnode;
+    ndims[3] = ffl.nvecs();
+    ndims[4] = 0;
+    ndims[5] = 0;
+    DOFs = ffl.apos;
+    fDOFs.assign(DOFs.size(), 0.0);
+    vDOFs.assign(DOFs.size(), 0.0);
+    tmpl_neighs = ffl.neighs;
+    selection.assign(std::max(ffl.natoms, 1), 0);
+}
+
+// pack one system (double, MMFFsp3_loc layout) into the float mirrors
+void World::pack_system(int s, bool bParams, bool bForces, bool bVel, bool blvec) {
+    const fc::MolSys& m = mols[s];
+    const fcd::SysInfo& I = info[s];
+    const int N = m.natoms, M = m.nnode, V = N + M;
+    for (int v = 0; v < V; v++) atoms[I.v0 + v] = f4(m.apos[v * 3], m.apos[v * 3 + 1], m.apos[v * 3 + 2], 0.0);
+    if (bForces && (int)m.fapos.size() == V * 3)
+        for (int v = 0; v < V; v++) aforces[I.v0 + v] = f4(m.fapos[v * 3], m.fapos[v * 3 + 1], m.fapos[v * 3 + 2], 0.0);
+    if (bVel && (int)m.vapos.size() == V * 3)
+        for (int v = 0; v < V; v++) avel[I.v0 + v] = f4(m.vapos[v * 3], m.vapos[v * 3 + 1], m.vapos[v * 3 + 2], 0.0);
+    if (blvec) {
+        // vacuum / non-periodic molecules: zero lattice, one zero PBC shift
+        for (int k = 0; k < 3; k++) {
+            lvecs[s * 3 + k] = f4(0, 0, 0, 0);
+            ilvecs[s * 3 + k] = f4(0, 0, 0, 0);
+        }
+        pbcshifts[s] = f4(0, 0, 0, 0);
+    }
+    if (!bParams) return;
+    const double alpha = grid.alphaMorse;
+    for (int i = 0; i < N; i++) {
+        const double* q = &m.REQs[i * 4];
+        REQs[I.a0 + i] = f4(q[0], q[1], q[2], q[3]);
+        // REQ2PLQ_d (Forces.h:611-619) in double, rounded once to float
+        const double e = std::exp(alpha * q[0]);
+        const double cL = e * q[1];
+        PLQs[I.a0 + i] = f4(e * cL, cL, q[2], e * e * q[3]);
+        neighs[I.a0 + i] = make_int4(m.neighs[i * 4], m.neighs[i * 4 + 1], m.neighs[i * 4 + 2], m.neighs[i * 4 + 3]);
+        neighCell[I.a0 + i] = make_int4(-1, -1, -1, -1);
+        elem[I.a0 + i] = m.elem.empty() ? 0 : m.elem[i];
+    }
+    for (int k = 0; k < M; k++) {
+        auto P4 = [&](const std::vector<double>& a) { return f4(a[k * 4], a[k * 4 + 1], a[k * 4 + 2], a[k * 4 + 3]); };
+        MMpars[I.n0 + k] = P4(m.apars);
+        BLs[I.n0 + k] = P4(m.bLs);
+        BKs[I.n0 + k] = P4(m.bKs);
+        Ksp[I.n0 + k] = P4(m.Ksp);
+        Kpp[I.n0 + k] = P4(m.Kpp);
+        node_sys[I.n0 + k] = s;
+    }
+    std::vector<int> bk((size_t)N * 4);
+    if (!fc::make_back_neighs(N, M, m.neighs.data(), bk.data()))
+        std::fprintf(stderr, "[libMMFFmulti_lib] system %d: an atom has more than 4 back-neighbours\n", s);
+    const int base = I.n0 * 8;
+    auto add = [&](int x, int off) { return x < 0 ? -1 : x + off; };
+    for (int i = 0; i < N; i++)
+        bkNeighs[I.v0 + i] = make_int4(add(bk[i * 4], base), add(bk[i * 4 + 1], base), add(bk[i * 4 + 2], base),
+                                       add(bk[i * 4 + 3], base));
+    for (int k = 0; k < M; k++)
+        bkNeighs[I.v0 + N + k] = make_int4(add(bk[k * 4], base + 4 * M), add(bk[k * 4 + 1], base + 4 * M),
+                                           add(bk[k * 4 + 2], base + 4 * M), add(bk[k * 4 + 3], base + 4 * M));
+}
+
+void World::unpack_system(int s, bool bForces, bool bVel) {
+    fc::MolSys& m = mols[s];
+    const fcd::SysInfo& I = info[s];
+    const int V = m.natoms + m.nnode;
+    for (int v = 0; v < V; v++) {
+        m.apos[v * 3] = atoms[I.v0 + v].x;
+        m.apos[v * 3 + 1] = atoms[I.v0 + v].y;
+        m.apos[v * 3 + 2] = atoms[I.v0 + v].z;
+    }
+    if (bForces) {
+        m.fapos.resize((size_t)V * 3);
+        for (int v = 0; v < V; v++) {
+            m.fapos[v * 3] = aforces[I.v0 + v].x;
+            m.fapos[v * 3 + 1] = aforces[I.v0 + v].y;
+            m.fapos[v * 3 + 2] = aforces[I.v0 + v].z;
+        }
+    }
+    if (bVel) {
+        m.vapos.resize((size_t)V * 3);
+        for (int v = 0; v < V; v++) {
+            m.vapos[v * 3] = avel[I.v0 + v].x;
+            m.vapos[v * 3 + 1] = avel[I.v0 + v].y;
+            m.vapos[v * 3 + 2] = avel[I.v0 + v].z;
+        }
+    }
+}
+
+bool World::ensure_device() {
+    if (dev_ready) return true;
+    int ndev = 0;
+    if (!hip_ok(hipGetDeviceCount(&ndev), "hipGetDeviceCount") || ndev < 1) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] no HIP device available: this library has no CPU path\n");
+        return false;
+    }
+    if (!hip_ok(hipStreamCreate(&stream), "hipStreamCreate")) return false;
+    for (int i = 0; i < 4; i++) {
+        hipStream_t st;
+        if (!hip_ok(hipStreamCreate(&st), "hipStreamCreate")) return false;
+        class_streams.push_back(st);
+    }
+    dev_ready = true;
+    return true;
+}
+
+bool World::upload_all(bool bParams, bool bForces, bool bVel) {
+    if (!ensure_device()) return false;
+    d_atoms.upload(atoms, stream);
+    d_atoms0.upload(atoms, stream);
+    if (bForces || !d_aforce.p) d_aforce.upload(aforces, stream);
+    if (bVel || !d_avel.p) d_avel.upload(avel, stream);
+    if (bParams || !d_REQs.p) {
+        d_REQs.upload(REQs, stream);
+        d_PLQs.upload(PLQs, stream);
+        d_neighs.upload(neighs, stream);
+        d_bkNeighs.upload(bkNeighs, stream);
+        d_apars.upload(MMpars, stream);
+        d_bLs.upload(BLs, stream);
+        d_bKs.upload(BKs, stream);
+        d_Ksp.upload(Ksp, stream);
+        d_Kpp.upload(Kpp, stream);
+        d_excl.upload(excl, stream);
+        d_corr.upload(corr, stream);
+        d_info.upload(info, stream);
+        d_node_sys.upload(node_sys, stream);
+        d_fneigh.alloc(std::max<size_t>((size_t)ntotM * 8, 1));
+        d_ebond.alloc(std::max<size_t>((size_t)ntotM, 1));
+        d_fire.alloc(std::max(nSys, 1));
+        d_nsteps.upload(nsteps, stream);
+        d_status.upload(status, stream);
+        d_Esys.upload(Esys, stream);
+        d_Fmax.upload(Fmax, stream);
+        d_queue.alloc(8);
+        plan_classes();
+    }
+    if (bGridFF && !grid_f4.empty() && d_grid.n != grid_f4.size()) d_grid.upload(grid_f4, stream);
+    return hip_ok(hipStreamSynchronize(stream), "upload");
+}
+
+bool World::download_all(bool bForces, bool bVel) {
+    if (!dev_ready) return false;
+    d_atoms.download(atoms, stream);
+    if (bForces) d_aforce.download(aforces, stream);
+    if (bVel) d_avel.download(avel, stream);
+    d_nsteps.download(nsteps, stream);
+    d_status.download(status, stream);
+    d_Esys.download(Esys, stream);
+    d_Fmax.download(Fmax, stream);
+    bool ok = hip_ok(hipStreamSynchronize(stream), "download");
+    // refresh the template views from system 0
+    if (ok && nSys > 0) {
+        const fcd::SysInfo& I = info[0];
+        const int V = I.natoms + I.nnode;
+        for (int v = 0; v < V; v++) {
+            DOFs[v * 3] = atoms[I.v0 + v].x;
+            DOFs[v * 3 + 1] = atoms[I.v0 + v].y;
+            DOFs[v * 3 + 2] = atoms[I.v0 + v].z;
+            fDOFs[v * 3] = aforces[I.v0 + v].x;
+            fDOFs[v * 3 + 1] = aforces[I.v0 + v].y;
+            fDOFs[v * 3 + 2] = aforces[I.v0 + v].z;
+            vDOFs[v * 3] = avel[I.v0 + v].x;
+            vDOFs[v * 3 + 1] = avel[I.v0 + v].y;
+            vDOFs[v * 3 + 2] = avel[I.v0 + v].z;
+        }
+        Es[0] = Esys[0];
+    }
+    return ok;
+}
+
+bool World::set_grid(const fc::GridFFHost& g) {
+    grid = g;
+    const size_t np = (size_t)g.nx * g.ny * g.nz;
+    grid_f4.resize(np);
+    // Bspline_to_f4 (GridFF.h:1219-1227)
+    for (size_t i = 0; i < np; i++)
+        grid_f4[i] = f4(g.coefs[i * 3], g.coefs[i * 3 + 1], g.coefs[i * 3 + 2], 0.0);
+    bGridFF = true;
+    // PLQs depend on alphaMorse
+    for (int s = 0; s < nSys; s++) pack_system(s, true, false, false, false);
+    if (dev_ready) {
+        d_grid.upload(grid_f4, stream);
+        d_PLQs.upload(PLQs, stream);
+        return hip_ok(hipStreamSynchronize(stream), "set_grid");
+    }
+    return true;
+}
+
+fcd::BatchDev World::batch_dev() {
+    fcd::BatchDev B{};
+    B.nsys = nSys;
+    B.info = d_info.p;
+    B.atoms = d_atoms.p;
+    B.avel = d_avel.p;
+    B.aforce = d_aforce.p;
+    B.REQs = d_REQs.p;
+    B.PLQs = d_PLQs.p;
+    B.neighs = d_neighs.p;
+    B.bkNeighs = d_bkNeighs.p;
+    B.apars = d_apars.p;
+    B.bLs = d_bLs.p;
+    B.bKs = d_bKs.p;
+    B.Ksp = d_Ksp.p;
+    B.Kpp = d_Kpp.p;
+    B.excl = d_excl.p;
+    B.corr = d_corr.p;
+    B.fneigh = d_fneigh.p;
+    B.ebond = d_ebond.p;
+    B.nsteps = d_nsteps.p;
+    B.status = d_status.p;
+    B.Esys = d_Esys.p;
+    B.Fmax = d_Fmax.p;
+    B.fire = d_fire.p;
+    return B;
+}
+
+fcd::RunParams World::run_params(double Fconv, int nstep_max) {
+    fcd::RunParams P{};
+    P.sw = sw;
+    if (!bMMFF) P.sw = fcd::FFSwitch{0, 0, 0, 0};
+    P.bNonBonded = bNonBonded ? 1 : 0;
+    P.bGridFF = (bGridFF && d_grid.p) ? 1 : 0;
+    P.grid.coefs = d_grid.p;
+    P.grid.nx = grid.nx;
+    P.grid.ny = grid.ny;
+    P.grid.nz = grid.nz;
+    P.grid.p0x = (float)grid.pos0[0];
+    P.grid.p0y = (float)grid.pos0[1];
+    P.grid.p0z = (float)grid.pos0[2];
+    P.grid.isx = (float)(1.0 / grid.step[0]);
+    P.grid.isy = (float)(1.0 / grid.step[1]);
+    P.grid.isz = (float)(1.0 / grid.step[2]);
+    P.fire = fp;
+    P.Fconv2 = (float)(Fconv * Fconv);
+    P.conv_mode = conv_mode;
+    P.nstep_max = nstep_max;
+    return P;
+}
+
+// Group systems by workgroup shape and order them by decreasing cost (LPT),
+// then split the GPU's resident workgroup slots between the classes in
+// proportion to their work so that all classes finish together.
+void World::plan_classes() {
+    classes.clear();
+    const int blocks[3] = {64, 128, 256};
+    std::vector<SizeClass> cl(3);
+    for (int c = 0; c < 3; c++) cl[c].block = blocks[c];
+    std::vector<int> too_big;
+    for (int s = 0; s < nSys; s++) {
+        const int N = info[s].natoms, M = info[s].nnode;
+        int c = -1;
+        for (int k = 0; k < 3; k++)
+            if (N <= blocks[k] && M <= blocks[k] && N + M <= 2 * blocks[k]) {
+                c = k;
+                break;
+            }
+        if (c < 0) {
+            too_big.push_back(s);
+            continue;
+        }
+        SizeClass& C = cl[c];
+        C.order.push_back(s);
+        C.nmax = std::max(C.nmax, N);
+        C.mmax = std::max(C.mmax, M);
+        C.work += (double)N * N + 40.0 * N;
+    }
+    if (!too_big.empty())
+        std::fprintf(stderr, "[libMMFFmulti_lib] %zu systems exceed 256 atoms and are relaxed by the split path only\n",
+                     too_big.size());
+    int ncu = 256;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) == hipSuccess) ncu = prop.multiProcessorCount;
+    }
+    double wtot = 0;
+    for (auto& C : cl) wtot += C.work;
+    for (auto& C : cl) {
+        if (C.order.empty()) continue;
+        C.wmax = (C.nmax + 31) / 32;
+        C.lds = (size_t)fc_lds_bytes(C.nmax, C.mmax, C.wmax);
+        std::sort(C.order.begin(), C.order.end(), [&](int a, int b) {
+            double ca = (double)info[a].natoms * info[a].natoms, cb = (double)info[b].natoms * info[b].natoms;
+            return ca != cb ? ca > cb : a < b;
+        });
+        int nb = 1;
+        fc_persistent_occupancy(C.block, C.lds, &nb);
+        nb = std::max(nb, 1);
+        const int slots = nb * ncu;
+        int g = (int)std::lround(slots * C.work / std::max(wtot, 1e-30));
+        g = std::max(1, std::min(g, (int)C.order.size()));
+        C.ngroups = g;
+        classes.push_back(C);
+    }
+    // device copy of the concatenated orders
+    std::vector<int> allorder;
+    for (auto& C : classes) allorder.insert(allorder.end(), C.order.begin(), C.order.end());
+    if (allorder.empty()) allorder.push_back(0);
+    d_order.upload(allorder, stream);
+}
+
+int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
+    if (!dev_ready || nSys == 0) return -1;
+    if (restore_start)
+        hip_ok(hipMemcpyAsync(d_atoms.p, d_atoms0.p, sizeof(float4) * ntotV, hipMemcpyDeviceToDevice, stream),
+               "restore");
+    fcd::BatchDev B = batch_dev();
+    fcd::RunParams P = run_params(Fconv, nstep_max);
+    std::vector<int> zeros(8, 0);
+    hip_ok(hipMemcpyAsync(d_queue.p, zeros.data(), sizeof(int) * 8, hipMemcpyHostToDevice, stream), "queue reset");
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, stream);
+    std::vector<hipEvent_t> done;
+    int off = 0;
+    for (size_t c = 0; c < classes.size(); c++) {
+        SizeClass& C = classes[c];
+        hipStream_t st = class_streams[c % class_streams.size()];
+        hipStreamWaitEvent(st, e0, 0);
+        hipError_t err = fc_launch_relax_persistent(C.block, C.ngroups, C.lds, st, &B, &P, d_order.p + off,
+                                                    (int)C.order.size(), d_queue.p + c, C.nmax, C.mmax, C.wmax);
+        if (!hip_ok(err, "relax_persistent launch")) return -1;
+        hipEvent_t ev;
+        hipEventCreate(&ev);
+        hipEventRecord(ev, st);
+        hipStreamWaitEvent(stream, ev, 0);
+        done.push_back(ev);
+        off += (int)C.order.size();
+    }
+    hipEventRecord(e1, stream);
+    if (!hip_ok(hipStreamSynchronize(stream), "relax_persistent")) return -1;
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    last_ms = ms;
+    for (auto ev : done) hipEventDestroy(ev);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    d_nsteps.download(nsteps, stream);
+    d_status.download(status, stream);
+    hip_ok(hipStreamSynchronize(stream), "relax results");
+    int mx = 0;
+    for (int s = 0; s < nSys; s++) mx = std::max(mx, nsteps[s]);
+    return mx;
+}
+
+int World::relax_split(int nstep_max, double Fconv, bool restore_start) {
+    if (!dev_ready || nSys == 0) return -1;
+    if (restore_start)
+        hip_ok(hipMemcpyAsync(d_atoms.p, d_atoms0.p, sizeof(float4) * ntotV, hipMemcpyDeviceToDevice, stream),
+               "restore");
+    fcd::BatchDev B = batch_dev();
+    fcd::RunParams P = run_params(Fconv, nstep_max);
+    hipMemsetAsync(d_avel.p, 0, sizeof(float4) * ntotV, stream);
+    fc_launch_reset(stream, &B, &P);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, stream);
+    const int chunk = 50;
+    int step = 0;
+    for (; step <= nstep_max;) {
+        int n = std::min(chunk, nstep_max + 1 - step);
+        for (int k = 0; k < n; k++, step++)
+            if (!hip_ok(fc_launch_split_step(stream, &B, &P, d_node_sys.p, ntotM, maxN, 1, step), "split step"))
+                return -1;
+        d_status.download(status, stream);
+        hip_ok(hipStreamSynchronize(stream), "split status");
+        bool any = false;
+        for (int s = 0; s < nSys && !any; s++) any = (status[s] == fcd::ST_RUNNING);
+        if (!any) break;
+    }
+    hipEventRecord(e1, stream);
+    hipStreamSynchronize(stream);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    last_ms = ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    d_nsteps.download(nsteps, stream);
+    d_status.download(status, stream);
+    hip_ok(hipStreamSynchronize(stream), "split results");
+    int mx = 0;
+    for (int s = 0; s < nSys; s++) mx = std::max(mx, nsteps[s]);
+    return mx;
+}
+
+bool World::eval_forces() {
+    if (!dev_ready || nSys == 0) return false;
+    fcd::BatchDev B = batch_dev();
+    fcd::RunParams P = run_params(0.0, 0);
+    fc_launch_reset(stream, &B, &P);
+    if (!hip_ok(fc_launch_split_step(stream, &B, &P, d_node_sys.p, ntotM, maxN, 0, 0), "eval_forces")) return false;
+    return hip_ok(hipStreamSynchronize(stream), "eval_forces sync");
+}
+
+bool World::sync() { return dev_ready ? hip_ok(hipStreamSynchronize(stream), "sync") : true; }
+
+void World::register_buffers() {
+    buffers.clear();
+    fbuffers.clear();
+    ibuffers.clear();
+    // template system (MMFFmulti_lib.cpp:23-71): apos/pipos alias DOFs
+    buffers["DOFs"] = DOFs.data();
+    buffers["fDOFs"] = fDOFs.data();
+    buffers["vDOFs"] = vDOFs.data();
+    buffers["apos"] = DOFs.data();
+    buffers["fapos"] = fDOFs.data();
+    buffers["pipos"] = DOFs.data() + (size_t)ffl.natoms * 3;
+    buffers["fpipos"] = fDOFs.data() + (size_t)ffl.natoms * 3;
+    buffers["Es"] = Es;
+    ibuffers["neighs"] = tmpl_neighs.data();
+    ibuffers["ndims"] = ndims;
+    ibuffers["selection"] = selection.data();
+    ibuffers["gpu_neighs"] = (int*)neighs.data();
+    ibuffers["gpu_neighCell"] = (int*)neighCell.data();
+    ibuffers["gpu_bkNeighs"] = (int*)bkNeighs.data();
+    fbuffers["gpu_atoms"] = (float*)atoms.data();
+    fbuffers["gpu_aforces"] = (float*)aforces.data();
+    fbuffers["gpu_avel"] = (float*)avel.data();
+    fbuffers["gpu_constr"] = (float*)constr.data();
+    fbuffers["gpu_REQs"] = (float*)REQs.data();
+    fbuffers["gpu_MMpars"] = (float*)MMpars.data();
+    fbuffers["gpu_BLs"] = (float*)BLs.data();
+    fbuffers["gpu_BKs"] = (float*)BKs.data();
+    fbuffers["gpu_Ksp"] = (float*)Ksp.data();
+    fbuffers["gpu_Kpp"] = (float*)Kpp.data();
+    fbuffers["gpu_lvecs"] = (float*)lvecs.data();
+    fbuffers["gpu_ilvecs"] = (float*)ilvecs.data();
+    fbuffers["gpu_pbcshifts"] = (float*)pbcshifts.data();
+    // additive views of the uneven (CSR) batch
+    ibuffers["sys_info"] = (int*)info.data();
+    ibuffers["sys_nsteps"] = nsteps.data();
+    ibuffers["sys_status"] = status.data();
+    ibuffers["gpu_elem"] = elem.data();
+    fbuffers["gpu_PLQs"] = (float*)PLQs.data();
+    fbuffers["sys_Fmax"] = Fmax.data();
+    buffers["sys_E"] = Esys.data();
+}
+
+}  // namespace fcw

@@ -1,0 +1,263 @@
+"""Loader and typed bindings of libMMFFmulti_lib.so (the HIP library).
+
+The library is built in-tree by ``firecore_amd.build``; loading fails loudly
+when it is missing -- there is no Python or CPU fallback for any compute.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libMMFFmulti_lib.so")
+
+c_int, c_double, c_bool, c_char_p, c_void_p, c_longlong = (ctypes.c_int, ctypes.c_double, ctypes.c_bool,
+                                                           ctypes.c_char_p, ctypes.c_void_p, ctypes.c_longlong)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_int_p = ctypes.POINTER(ctypes.c_int)
+
+_lib = None
+
+# status codes written per system
+ST_RUNNING, ST_CONVERGED, ST_MAXSTEP, ST_NAN = 0, 1, 2, 3
+# system-info record (int32 x 8): natoms, nnode, a0, n0, v0, mask0, corr0, ncorr
+SYSINFO_FIELDS = ("natoms", "nnode", "a0", "n0", "v0", "mask0", "corr0", "ncorr")
+
+
+def _p(a, t=c_double_p):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libMMFFmulti_lib.so is not built (run `python -m firecore_amd.build`); "
+                           "this package has no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
+    L.init_workload.argtypes = [c_int, c_int, c_longlong, c_int, c_int, c_double, c_char_p, c_double]
+    L.init_workload.restype = c_int
+    L.init_systems.argtypes = [c_int, c_int_p, c_int_p, c_double_p, c_double_p, c_int_p, c_double_p, c_double_p,
+                               c_double_p, c_double_p, c_double_p, c_int_p]
+    L.init_systems.restype = c_int
+    L.relax.argtypes = [c_int, c_double, c_int, c_bool, c_bool]
+    L.relax.restype = c_int
+    L.get_last_ms.restype = c_double
+    L.eval_forces.restype = c_int
+    L.set_conv_mode.argtypes = [c_int]
+    L.get_batch_sizes.argtypes = [c_int_p]
+    L.get_grid_info.argtypes = [c_double_p]
+    L.get_grid_info.restype = c_int
+    L.get_grid_coefs.restype = c_double_p
+    L.gen_molecule_arrays.argtypes = [c_int, c_longlong, c_double, c_longlong, c_int_p] + [c_void_p] * 9
+    L.gen_molecule_arrays.restype = c_int
+    L.make_back_neighs.argtypes = [c_int, c_int, c_int_p, c_int_p]
+    L.make_back_neighs.restype = c_int
+    L.make_exclusions.argtypes = [c_int, c_int, c_int_p, c_int, ctypes.POINTER(ctypes.c_uint), c_int_p, c_int_p]
+    L.make_exclusions.restype = c_int
+    L.make_nacl_gridff.argtypes = [c_int, c_int, c_int, c_double, c_int, c_double_p, c_double_p]
+    L.make_nacl_gridff.restype = c_int
+    L.getfBuff.argtypes = [c_char_p]
+    L.getfBuff.restype = c_float_p
+    L.getIBuff.argtypes = [c_char_p]
+    L.getIBuff.restype = c_int_p
+    L.getBuff.argtypes = [c_char_p]
+    L.getBuff.restype = c_double_p
+    L.set_opt.argtypes = [c_double, c_double, c_double, c_double, c_double, c_double, c_int, c_double, c_double]
+    L.set_opt.restype = None
+    L.setSwitches_multi.argtypes = [c_int] * 9
+    L.setSwitches_multi.restype = None
+    _lib = L
+    return L
+
+
+# ------------------------------------------------------------ host helpers
+def gen_molecule(natoms, seed, sigma=0.0, pseed=0):
+    """Synthetic molecule (host generator; no device work) as a dict of arrays
+    in the MMFFsp3_loc layout."""
+    L = lib()
+    n = natoms
+    nn = c_int(0)
+    apos = np.zeros((2 * n, 3))
+    REQs = np.zeros((n, 4))
+    neighs = np.zeros((n, 4), np.int32)
+    pars = [np.zeros((n, 4)) for _ in range(5)]
+    elem = np.zeros(n, np.int32)
+    r = L.gen_molecule_arrays(n, seed, sigma, pseed, ctypes.byref(nn), apos.ctypes.data, REQs.ctypes.data,
+                              neighs.ctypes.data, *[p.ctypes.data for p in pars], elem.ctypes.data)
+    if r < 0:
+        raise ValueError("cannot generate a %d-atom molecule" % natoms)
+    m = nn.value
+    d = dict(natoms=n, nnode=m, apos=np.ascontiguousarray(apos[:n + m]), REQs=REQs, neighs=neighs, elem=elem)
+    for k, p in zip(("apars", "bLs", "bKs", "Ksp", "Kpp"), pars):
+        d[k] = np.ascontiguousarray(p[:m])
+    return d
+
+
+def make_back_neighs(natoms, nnode, neighs):
+    neighs = np.ascontiguousarray(neighs, np.int32)
+    bk = np.zeros((natoms, 4), np.int32)
+    r = lib().make_back_neighs(natoms, nnode, _p(neighs, c_int_p), _p(bk, c_int_p))
+    return r, bk
+
+
+def make_exclusions(natoms, nnode, neighs, subtract13=True):
+    neighs = np.ascontiguousarray(neighs, np.int32)
+    W = (natoms + 31) // 32
+    mask = np.zeros((natoms, W), np.uint32)
+    w = c_int(0)
+    nc = c_int(0)
+    lib().make_exclusions(natoms, nnode, _p(neighs, c_int_p), int(subtract13),
+                          mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), ctypes.byref(w), ctypes.byref(nc))
+    bits = np.unpackbits(mask.view(np.uint8), bitorder="little").reshape(natoms, W * 32)[:, :natoms]
+    return bits.astype(bool), nc.value
+
+
+def make_nacl_grid(nx=56, ny=56, nz=200, dz=0.1, nthreads=8):
+    """Synthetic NaCl(001) GridFF coefficients (host direct sums + fit)."""
+    coefs = np.zeros((nx, ny, nz, 3))
+    geom = np.zeros(7)
+    lib().make_nacl_gridff(nx, ny, nz, dz, nthreads, _p(coefs), _p(geom))
+    return dict(n=(nx, ny, nz), pos0=geom[0:3].copy(), step=geom[3:6].copy(), alpha=float(geom[6]), coefs=coefs)
+
+
+# ------------------------------------------------------------ batch API
+def init_workload(kind, nsys, seed0, nmin, nmax, sigma=0.1, surf=None, grid_step=0.1):
+    n = lib().init_workload(kind, nsys, seed0, nmin, nmax, sigma, surf.encode() if surf else None, grid_step)
+    if n < 0:
+        raise RuntimeError("init_workload failed")
+    return n
+
+
+def init_systems(systems):
+    """Upload a list of molecule dicts (gen_molecule layout) as one batch."""
+    cat = lambda k, dt=np.float64: np.ascontiguousarray(np.concatenate([np.asarray(m[k], dt).reshape(-1) for m in systems]))
+    natoms = np.array([m["natoms"] for m in systems], np.int32)
+    nnode = np.array([m["nnode"] for m in systems], np.int32)
+    arrs = [cat("apos"), cat("REQs"), cat("neighs", np.int32), cat("apars"), cat("bLs"), cat("bKs"), cat("Ksp"),
+            cat("Kpp"), cat("elem", np.int32)]
+    n = lib().init_systems(len(systems), _p(natoms, c_int_p), _p(nnode, c_int_p), _p(arrs[0]), _p(arrs[1]),
+                           _p(arrs[2], c_int_p), _p(arrs[3]), _p(arrs[4]), _p(arrs[5]), _p(arrs[6]), _p(arrs[7]),
+                           _p(arrs[8], c_int_p))
+    if n < 0:
+        raise RuntimeError("init_systems failed")
+    return n
+
+
+def batch_sizes():
+    out = np.zeros(9, np.int32)
+    lib().get_batch_sizes(_p(out, c_int_p))
+    keys = ("nsys", "ntotA", "ntotM", "ntotV", "maxN", "maxM", "gnx", "gny", "gnz")
+    return dict(zip(keys, (int(x) for x in out)))
+
+
+def grid_info():
+    g = np.zeros(7)
+    n = lib().get_grid_info(_p(g))
+    s = batch_sizes()
+    if n <= 0:
+        return None
+    ptr = lib().get_grid_coefs()
+    coefs = np.ctypeslib.as_array(ptr, shape=(s["gnx"], s["gny"], s["gnz"], 3))
+    return dict(n=(s["gnx"], s["gny"], s["gnz"]), pos0=g[0:3].copy(), step=g[3:6].copy(), alpha=float(g[6]),
+                coefs=coefs)
+
+
+def fview(name, n):
+    ptr = lib().getfBuff(name.encode())
+    if not ptr:
+        raise KeyError(name)
+    return np.ctypeslib.as_array(ptr, shape=(n,))
+
+
+def iview(name, n):
+    ptr = lib().getIBuff(name.encode())
+    if not ptr:
+        raise KeyError(name)
+    return np.ctypeslib.as_array(ptr, shape=(n,))
+
+
+def dview(name, n):
+    ptr = lib().getBuff(name.encode())
+    if not ptr:
+        raise KeyError(name)
+    return np.ctypeslib.as_array(ptr, shape=(n,))
+
+
+class Batch:
+    """Numpy views of the host mirrors of the current CSR-packed batch."""
+
+    def __init__(self):
+        s = batch_sizes()
+        self.sizes = s
+        ns, A, M, V = s["nsys"], s["ntotA"], s["ntotM"], s["ntotV"]
+        self.nsys = ns
+        self.info = iview("sys_info", ns * 8).reshape(ns, 8)
+        self.atoms = fview("gpu_atoms", V * 4).reshape(V, 4)
+        self.aforces = fview("gpu_aforces", V * 4).reshape(V, 4)
+        self.avel = fview("gpu_avel", V * 4).reshape(V, 4)
+        self.REQs = fview("gpu_REQs", A * 4).reshape(A, 4)
+        self.PLQs = fview("gpu_PLQs", A * 4).reshape(A, 4)
+        self.neighs = iview("gpu_neighs", A * 4).reshape(A, 4)
+        self.bkNeighs = iview("gpu_bkNeighs", V * 4).reshape(V, 4)
+        self.elem = iview("gpu_elem", A)
+        self.MMpars = fview("gpu_MMpars", M * 4).reshape(M, 4)
+        self.BLs = fview("gpu_BLs", M * 4).reshape(M, 4)
+        self.BKs = fview("gpu_BKs", M * 4).reshape(M, 4)
+        self.Ksp = fview("gpu_Ksp", M * 4).reshape(M, 4)
+        self.Kpp = fview("gpu_Kpp", M * 4).reshape(M, 4)
+        self.nsteps = iview("sys_nsteps", ns)
+        self.status = iview("sys_status", ns)
+        self.Fmax = fview("sys_Fmax", ns)
+        self.E = dview("sys_E", ns)
+
+    def system(self, s, double=True):
+        """System s as a molecule dict (gen_molecule layout) from the mirrors."""
+        N, M, a0, n0, v0 = (int(x) for x in self.info[s, :5])
+        d = dict(natoms=N, nnode=M,
+                 apos=self.atoms[v0:v0 + N + M, :3].astype(np.float64),
+                 REQs=self.REQs[a0:a0 + N].astype(np.float64),
+                 neighs=np.ascontiguousarray(self.neighs[a0:a0 + N]),
+                 elem=np.ascontiguousarray(self.elem[a0:a0 + N]),
+                 apars=self.MMpars[n0:n0 + M].astype(np.float64),
+                 bLs=self.BLs[n0:n0 + M].astype(np.float64),
+                 bKs=self.BKs[n0:n0 + M].astype(np.float64),
+                 Ksp=self.Ksp[n0:n0 + M].astype(np.float64),
+                 Kpp=self.Kpp[n0:n0 + M].astype(np.float64))
+        return d
+
+    def forces(self, s):
+        N, M, a0, n0, v0 = (int(x) for x in self.info[s, :5])
+        return self.aforces[v0:v0 + N + M, :3].astype(np.float64)
+
+    def positions(self, s):
+        N, M, a0, n0, v0 = (int(x) for x in self.info[s, :5])
+        return self.atoms[v0:v0 + N + M, :3].astype(np.float64)
+
+
+def relax(nstep_max, Fconv, path=0, restore_start=False, download=True):
+    n = lib().relax(nstep_max, Fconv, path, restore_start, download)
+    if n < 0:
+        raise RuntimeError("relax failed")
+    return n
+
+
+def last_ms():
+    return lib().get_last_ms()
+
+
+def eval_forces():
+    if lib().eval_forces() != 0:
+        raise RuntimeError("eval_forces failed")
+
+
+def set_conv_mode(mode):
+    lib().set_conv_mode(mode)
+
+
+def set_opt(dt_max=0.05, dt_min=0.005, damp_max=0.1, finc=1.1, fdec=0.5, falpha=0.8, minLastNeg=3,
+            cvf_min=-0.1, cvf_max=0.1):
+    lib().set_opt(dt_max, dt_min, damp_max, finc, fdec, falpha, minLastNeg, cvf_min, cvf_max)

@@ -1,0 +1,182 @@
+"""FP64 CPU oracle of the multi-system MMFF relaxation path (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+package, and only as the checker or the timed CPU baseline.  The product
+library (firecore_amd/libMMFFmulti_lib.so) never loads it.
+
+Two back-ends with one call signature:
+  * ``liboracle.so``      -- this repository's FP64 restatement (oracle.cpp)
+  * ``_ref/libfcref.so``  -- the reference's own CPU classes compiled from
+    /root/reference by oracle/Makefile (absent where the reference is absent)
+
+Parity pins of the restatement (see tests/test_oracle_*.py): bit-level
+agreement with _ref on random molecules (forces, recoils, energies, FIRE
+trajectories, back-neighbours), the golden fixtures under tests/golden/
+generated from _ref, and closed-form checks (finite differences of the energy,
+Newton's third law, B-spline partition of unity and linear reproduction).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libfcref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _p(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def build(with_ref=None):
+    """Compile liboracle.so (and _ref when the reference tree is present)."""
+    import subprocess
+    targets = ["all"]
+    if with_ref is None:
+        with_ref = os.path.isdir("/root/reference/cpp/common")
+    if with_ref:
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+class Backend:
+    """ctypes view of liboracle.so (prefix 'orc_') or _ref/libfcref.so ('ref_')."""
+
+    def __init__(self, path, prefix):
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+        L = self.lib
+        f = getattr(L, prefix + "eval")
+        f.restype = ctypes.c_double
+        f = getattr(L, prefix + "relax")
+        f.restype = ctypes.c_int
+        f = getattr(L, prefix + "grid_sample")
+        f.restype = ctypes.c_double
+
+    def _fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    @staticmethod
+    def _sys_args(m, sw):
+        sw = np.ascontiguousarray(np.asarray(sw, dtype=np.int32))
+        return [ctypes.c_int(m["natoms"]), ctypes.c_int(m["nnode"]), _p(m["REQs"]), _p(m["neighs"], _ip),
+                _p(m["apars"]), _p(m["bLs"]), _p(m["bKs"]), _p(m["Ksp"]), _p(m["Kpp"]), _p(sw, _ip)], sw
+
+    @staticmethod
+    def _grid_args(grid):
+        if grid is None:
+            return [ctypes.c_int(0)] * 3 + [None, None, None, ctypes.c_double(1.5)]
+        return [ctypes.c_int(grid["n"][0]), ctypes.c_int(grid["n"][1]), ctypes.c_int(grid["n"][2]),
+                _p(grid["pos0"]), _p(grid["step"]), _p(grid["coefs"]), ctypes.c_double(grid["alpha"])]
+
+    def eval(self, m, apos=None, sw=(1, 1, 1, 1, 1), grid=None):
+        """Assembled forces (nvecs,3), recoils (nnode,4,3) x2 and total energy."""
+        N, M = m["natoms"], m["nnode"]
+        apos = np.ascontiguousarray(m["apos"] if apos is None else apos, dtype=np.float64)
+        f = np.zeros((N + M, 3))
+        fng = np.zeros((M, 4, 3))
+        fngpi = np.zeros((M, 4, 3))
+        a, sw_keep = self._sys_args(m, sw)
+        args = a[:2] + [_p(apos)] + a[2:] + self._grid_args(grid) + [_p(f), _p(fng), _p(fngpi)]
+        E = self._fn("eval")(*args)
+        return f, fng, fngpi, E
+
+    def relax(self, m, fire, Fconv, nmax, conv_mode=0, sw=(1, 1, 1, 1, 1), grid=None, ntraj=0):
+        N, M = m["natoms"], m["nnode"]
+        apos = np.array(m["apos"], dtype=np.float64, copy=True)
+        fire = np.ascontiguousarray(np.asarray(fire, dtype=np.float64))
+        E = ctypes.c_double(0)
+        traj = np.zeros((max(ntraj, 1), N + M, 3))
+        a, sw_keep = self._sys_args(m, sw)
+        args = a[:2] + [_p(apos)] + a[2:] + self._grid_args(grid) + [
+            _p(fire), ctypes.c_double(Fconv), ctypes.c_int(nmax), ctypes.c_int(conv_mode), ctypes.byref(E),
+            _p(traj) if ntraj > 0 else None, ctypes.c_int(ntraj)]
+        n = self._fn("relax")(*args)
+        return n, apos, E.value, traj[:ntraj]
+
+    def make_back_neighs(self, natoms, nnode, neighs):
+        neighs = np.ascontiguousarray(neighs, dtype=np.int32)
+        bk = np.zeros((natoms, 4), np.int32)
+        ng = np.zeros((natoms, 4), np.int32)
+        r = self._fn("make_back_neighs")(ctypes.c_int(natoms), ctypes.c_int(nnode), _p(neighs, _ip), _p(bk, _ip),
+                                        _p(ng, _ip))
+        return r, bk, ng
+
+    def grid_sample(self, grid, p, plq):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        plq = np.ascontiguousarray(plq, dtype=np.float64)
+        f = np.zeros(3)
+        E = self._fn("grid_sample")(ctypes.c_int(grid["n"][0]), ctypes.c_int(grid["n"][1]),
+                                    ctypes.c_int(grid["n"][2]), _p(grid["pos0"]), _p(grid["step"]),
+                                    _p(grid["coefs"]), _p(p), _p(plq), _p(f))
+        return E, f
+
+    def sample_terms(self, kind, k, ang0, r1, r2, angles):
+        angles = np.ascontiguousarray(angles, dtype=np.float64)
+        n = len(angles)
+        E = np.zeros(n)
+        F = np.zeros(n)
+        self._fn("sample_terms")(ctypes.c_int(kind), ctypes.c_double(k), ctypes.c_double(ang0),
+                                 ctypes.c_double(r1), ctypes.c_double(r2), ctypes.c_int(n), _p(angles), _p(E), _p(F))
+        return E, F
+
+    def sample_pair(self, rs, REQij):
+        rs = np.ascontiguousarray(rs, dtype=np.float64)
+        REQij = np.ascontiguousarray(REQij, dtype=np.float64)
+        E = np.zeros(len(rs))
+        F = np.zeros(len(rs))
+        self._fn("sample_pair")(ctypes.c_int(len(rs)), _p(rs), _p(REQij), _p(E), _p(F))
+        return E, F
+
+
+def oracle():
+    return Backend(ORACLE_SO, "orc_")
+
+
+def reference():
+    """The reference build, or None when it was not built here."""
+    if not os.path.exists(REF_SO):
+        return None
+    return Backend(REF_SO, "ref_")
+
+
+def req2plq(REQ, alpha):
+    lib = ctypes.CDLL(ORACLE_SO)
+    out = np.zeros(4)
+    REQ = np.ascontiguousarray(REQ, dtype=np.float64)
+    lib.orc_req2plq(_p(REQ), ctypes.c_double(alpha), _p(out))
+    return out
+
+
+def relax_batch(systems, fire, Fconv, nmax, conv_mode=0, sw=(1, 1, 1, 1, 1), grid=None, nthreads=1):
+    """Relax a list of systems on `nthreads` host threads (FP64).  Returns
+    (steps, E, Fmax, positions list)."""
+    lib = ctypes.CDLL(ORACLE_SO)
+    ns = len(systems)
+    natoms = np.array([m["natoms"] for m in systems], np.int32)
+    nnode = np.array([m["nnode"] for m in systems], np.int32)
+    cat = lambda k, dt=np.float64: np.ascontiguousarray(np.concatenate([np.asarray(m[k], dt).reshape(-1) for m in systems]))
+    apos = cat("apos")
+    REQs, neighs = cat("REQs"), cat("neighs", np.int32)
+    apars, bLs, bKs, Ksp, Kpp = cat("apars"), cat("bLs"), cat("bKs"), cat("Ksp"), cat("Kpp")
+    sw = np.ascontiguousarray(np.asarray(sw, np.int32))
+    fire = np.ascontiguousarray(np.asarray(fire, np.float64))
+    steps = np.zeros(ns, np.int32)
+    E = np.zeros(ns)
+    Fm = np.zeros(ns)
+    g = Backend._grid_args(grid)
+    lib.orc_relax_batch(ctypes.c_int(ns), _p(natoms, _ip), _p(nnode, _ip), _p(apos), _p(REQs), _p(neighs, _ip),
+                        _p(apars), _p(bLs), _p(bKs), _p(Ksp), _p(Kpp), _p(sw, _ip), *g, _p(fire),
+                        ctypes.c_double(Fconv), ctypes.c_int(nmax), ctypes.c_int(conv_mode), ctypes.c_int(nthreads),
+                        _p(steps, _ip), _p(E), _p(Fm))
+    out = []
+    off = 0
+    for m in systems:
+        V = m["natoms"] + m["nnode"]
+        out.append(apos[off * 3:(off + V) * 3].reshape(V, 3))
+        off += V
+    return steps, E, Fm, out

@@ -1,0 +1,252 @@
+// Test-only driver around the reference's own CPU classes.
+//
+// This file is compiled by oracle/Makefile against the reference headers that
+// live under /root/reference/cpp/common (nothing of the reference is copied
+// here) into oracle/_ref/libfcref.so.  It calls the reference's code for
+// every physics/optimizer step so that the FP64 oracle (oracle/oracle.cpp)
+// can be pinned against it:
+//   MMFFsp3_loc::eval_atom / assemble_atom / makeBackNeighs / move_atom_FIRE
+//   NBFF::evalLJQs_ng4_atom_omp, GridFF::addAtom -> getForce_Bspline,
+//   REQ2PLQ_d, DynamicOpt::FIRE_update_params
+// The per-step sequence is the one of MolWorld_sp3::run_no_omp
+// (MolWorld_sp3.h:2291-2474) with the FIRE branch.
+#include "globals.h"
+#include "testUtils.h"
+#include "MMFFsp3_loc.h"
+#include "GridFF.h"
+#include "DynamicOpt.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace {
+
+struct RefSys {
+    MMFFsp3_loc ffl;
+    GridFF gff;
+    bool bGrid = false;
+    bool bNonBonded = true;
+    std::vector<Quat4d> PLQd;
+    DynamicOpt opt;
+
+    void setup(int natoms, int nnode, const double* apos, const double* REQs, const int* neighs,
+               const double* apars, const double* bLs, const double* bKs, const double* Ksp, const double* Kpp,
+               const int* sw) {
+        ffl.realloc(nnode, natoms - nnode);
+        ffl.bPBC = false;
+        for (int i = 0; i < ffl.nvecs; i++) ffl.apos[i] = Vec3d{apos[i * 3], apos[i * 3 + 1], apos[i * 3 + 2]};
+        for (int i = 0; i < natoms; i++) {
+            ffl.REQs[i] = Quat4d{REQs[i * 4], REQs[i * 4 + 1], REQs[i * 4 + 2], REQs[i * 4 + 3]};
+            ffl.neighs[i] = Quat4i{neighs[i * 4], neighs[i * 4 + 1], neighs[i * 4 + 2], neighs[i * 4 + 3]};
+            ffl.neighCell[i] = Quat4i{-1, -1, -1, -1};
+        }
+        for (int i = 0; i < nnode; i++) {
+            ffl.apars[i] = Quat4d{apars[i * 4], apars[i * 4 + 1], apars[i * 4 + 2], apars[i * 4 + 3]};
+            ffl.bLs[i] = Quat4d{bLs[i * 4], bLs[i * 4 + 1], bLs[i * 4 + 2], bLs[i * 4 + 3]};
+            ffl.bKs[i] = Quat4d{bKs[i * 4], bKs[i * 4 + 1], bKs[i * 4 + 2], bKs[i * 4 + 3]};
+            ffl.Ksp[i] = Quat4d{Ksp[i * 4], Ksp[i * 4 + 1], Ksp[i * 4 + 2], Ksp[i * 4 + 3]};
+            ffl.Kpp[i] = Quat4d{Kpp[i * 4], Kpp[i * 4 + 1], Kpp[i * 4 + 2], Kpp[i * 4 + 3]};
+        }
+        ffl.doBonds = sw[0];
+        ffl.doPiPiI = sw[1];
+        ffl.doPiSigma = sw[2];
+        ffl.doAngles = sw[3];
+        bNonBonded = sw[4];
+        ffl.bNonBonded = bNonBonded;
+        ffl.setNonBondStrategy(+1);  // bNonBondNeighs
+        ffl.makeBackNeighs();
+    }
+    void setup_grid(int nx, int ny, int nz, const double* pos0, const double* step, double* coefs, double alpha) {
+        bGrid = true;
+        gff.grid.n = Vec3i{nx, ny, nz};
+        gff.grid.pos0 = Vec3d{pos0[0], pos0[1], pos0[2]};
+        gff.grid.diCell = Mat3d{1.0 / step[0], 0, 0, 0, 1.0 / step[1], 0, 0, 0, 1.0 / step[2]};
+        gff.Bspline_PLQ = (Vec3d*)coefs;
+        gff.mode = GridFFmod::BsplineDouble;
+        make_inds_pbc(nx, gff.cubic_xqis);
+        make_inds_pbc(ny, gff.cubic_yqis);
+        gff.alphaMorse = alpha;
+        PLQd.resize(ffl.natoms);
+        for (int i = 0; i < ffl.natoms; i++) PLQd[i] = REQ2PLQ_d(ffl.REQs[i], alpha);
+    }
+    // MolWorld_sp3::eval_no_omp (MolWorld_sp3.h:2178-2245) recipe
+    double eval() {
+        double E = 0;
+        for (int i = 0; i < ffl.natoms; i++) ffl.fapos[i] = Vec3dZero;
+        for (int ia = 0; ia < ffl.natoms; ia++) {
+            ffl.fapos[ia] = Vec3dZero;
+            if (ia < ffl.nnode) ffl.fapos[ia + ffl.natoms] = Vec3dZero;
+            if (ia < ffl.nnode) E += ffl.eval_atom(ia);
+            if (bNonBonded) E += ffl.evalLJQs_ng4_atom_omp(ia);
+            if (bGrid) E += gff.addAtom(ffl.apos[ia], PLQd[ia], ffl.fapos[ia]);
+        }
+        for (int ia = 0; ia < ffl.natoms; ia++) ffl.assemble_atom(ia);
+        return E;
+    }
+};
+
+void copy_out(const RefSys& R, double* fapos, double* fneigh, double* fneighpi) {
+    if (fapos)
+        for (int i = 0; i < R.ffl.nvecs; i++) {
+            fapos[i * 3] = R.ffl.fapos[i].x;
+            fapos[i * 3 + 1] = R.ffl.fapos[i].y;
+            fapos[i * 3 + 2] = R.ffl.fapos[i].z;
+        }
+    if (fneigh)
+        for (int i = 0; i < R.ffl.nnode * 4; i++) {
+            fneigh[i * 3] = R.ffl.fneigh[i].x;
+            fneigh[i * 3 + 1] = R.ffl.fneigh[i].y;
+            fneigh[i * 3 + 2] = R.ffl.fneigh[i].z;
+        }
+    if (fneighpi)
+        for (int i = 0; i < R.ffl.nnode * 4; i++) {
+            fneighpi[i * 3] = R.ffl.fneighpi[i].x;
+            fneighpi[i * 3 + 1] = R.ffl.fneighpi[i].y;
+            fneighpi[i * 3 + 2] = R.ffl.fneighpi[i].z;
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_make_back_neighs(int natoms, int nnode, const int* neighs, int* bk_out, int* neighs_out) {
+    MMFFsp3_loc ffl;
+    ffl.realloc(nnode, natoms - nnode);
+    for (int i = 0; i < natoms; i++)
+        ffl.neighs[i] = Quat4i{neighs[i * 4], neighs[i * 4 + 1], neighs[i * 4 + 2], neighs[i * 4 + 3]};
+    ffl.makeBackNeighs();
+    for (int i = 0; i < natoms; i++) {
+        for (int k = 0; k < 4; k++) {
+            bk_out[i * 4 + k] = ffl.bkneighs[i].array[k];
+            if (neighs_out) neighs_out[i * 4 + k] = ffl.neighs[i].array[k];
+        }
+    }
+    return 0;
+}
+
+// Forces (assembled, atoms then pi), recoils and total energy of one system.
+double ref_eval(int natoms, int nnode, const double* apos, const double* REQs, const int* neighs, const double* apars,
+                const double* bLs, const double* bKs, const double* Ksp, const double* Kpp, const int* sw, int gnx,
+                int gny, int gnz, const double* gpos0, const double* gstep, double* gcoefs, double alpha,
+                double* fapos, double* fneigh, double* fneighpi) {
+    RefSys R;
+    R.setup(natoms, nnode, apos, REQs, neighs, apars, bLs, bKs, Ksp, Kpp, sw);
+    if (gcoefs) R.setup_grid(gnx, gny, gnz, gpos0, gstep, gcoefs, alpha);
+    double E = R.eval();
+    copy_out(R, fapos, fneigh, fneighpi);
+    if (gcoefs) R.gff.Bspline_PLQ = nullptr;
+    return E;
+}
+
+// B-spline GridFF sample of one point with coefficients PLQ (GridFF::addAtom).
+double ref_grid_sample(int gnx, int gny, int gnz, const double* gpos0, const double* gstep, double* gcoefs,
+                       const double* p, const double* plq, double* f) {
+    GridFF gff;
+    gff.grid.n = Vec3i{gnx, gny, gnz};
+    gff.grid.pos0 = Vec3d{gpos0[0], gpos0[1], gpos0[2]};
+    gff.grid.diCell = Mat3d{1.0 / gstep[0], 0, 0, 0, 1.0 / gstep[1], 0, 0, 0, 1.0 / gstep[2]};
+    gff.Bspline_PLQ = (Vec3d*)gcoefs;
+    gff.mode = GridFFmod::BsplineDouble;
+    make_inds_pbc(gnx, gff.cubic_xqis);
+    make_inds_pbc(gny, gff.cubic_yqis);
+    Vec3d fo = Vec3dZero;
+    double E = gff.addAtom(Vec3d{p[0], p[1], p[2]}, Quat4d{plq[0], plq[1], plq[2], plq[3]}, fo);
+    f[0] = fo.x;
+    f[1] = fo.y;
+    f[2] = fo.z;
+    gff.Bspline_PLQ = nullptr;
+    return E;
+}
+
+// FIRE relaxation of one system with the run_no_omp recipe.
+//   fire = {dt_max, dt_min, damp_max, finc, fdec, falpha, minLastNeg}
+//   conv_mode 0: stop before moving when max_i |f_i| < Fconv (all vecs)
+//   conv_mode 1: reference criterion, sum f^2 < Fconv^2 tested after the move
+// apos is updated in place; traj (optional) receives the positions after each
+// of the first ntraj steps.  Returns the step index at convergence (nmax if
+// not converged); E_out receives the energy of the last evaluation.
+int ref_relax(int natoms, int nnode, double* apos, const double* REQs, const int* neighs, const double* apars,
+              const double* bLs, const double* bKs, const double* Ksp, const double* Kpp, const int* sw, int gnx,
+              int gny, int gnz, const double* gpos0, const double* gstep, double* gcoefs, double alpha,
+              const double* fire, double Fconv, int nmax, int conv_mode, double* E_out, double* traj, int ntraj) {
+    RefSys R;
+    R.setup(natoms, nnode, apos, REQs, neighs, apars, bLs, bKs, Ksp, Kpp, sw);
+    if (gcoefs) R.setup_grid(gnx, gny, gnz, gpos0, gstep, gcoefs, alpha);
+    MMFFsp3_loc& ffl = R.ffl;
+    DynamicOpt& opt = R.opt;
+    opt.bindOrAlloc(ffl.nDOFs, ffl.DOFs, 0, ffl.fDOFs, 0);
+    opt.cleanVel();
+    ffl.vapos = (Vec3d*)opt.vel;
+    opt.dt = opt.dt_max = fire[0];
+    opt.dt_min = fire[1];
+    opt.damping = opt.damp_max = fire[2];
+    opt.finc = fire[3];
+    opt.fdec = fire[4];
+    opt.falpha = fire[5];
+    opt.minLastNeg = (int)fire[6];
+    opt.lastNeg = 0;
+    const double F2conv = Fconv * Fconv;
+    int itr = 0;
+    double E = 0;
+    for (itr = 0; itr < nmax; itr++) {
+        E = R.eval();
+        double vf = 0, vv = 0, ff = 0, fm2 = 0;
+        for (int i = 0; i < opt.n; i++) {
+            double v = opt.vel[i], f = opt.force[i];
+            vf += v * f;
+            vv += v * v;
+            ff += f * f;
+        }
+        for (int i = 0; i < ffl.nvecs; i++) fm2 = std::max(fm2, ffl.fapos[i].norm2());
+        if (conv_mode == 0 && fm2 < F2conv) break;
+        opt.vf = vf;
+        opt.vv = vv;
+        opt.ff = ff;
+        opt.FIRE_update_params();
+        for (int i = 0; i < ffl.nvecs; i++) ffl.move_atom_FIRE(i, opt.dt, 10000.0, opt.cv, opt.renorm_vf * opt.cf);
+        if (traj && itr < ntraj)
+            for (int i = 0; i < ffl.nvecs; i++) {
+                traj[((size_t)itr * ffl.nvecs + i) * 3] = ffl.apos[i].x;
+                traj[((size_t)itr * ffl.nvecs + i) * 3 + 1] = ffl.apos[i].y;
+                traj[((size_t)itr * ffl.nvecs + i) * 3 + 2] = ffl.apos[i].z;
+            }
+        if (conv_mode == 1 && ff < F2conv) break;
+    }
+    for (int i = 0; i < ffl.nvecs; i++) {
+        apos[i * 3] = ffl.apos[i].x;
+        apos[i * 3 + 1] = ffl.apos[i].y;
+        apos[i * 3 + 2] = ffl.apos[i].z;
+    }
+    if (E_out) *E_out = E;
+    if (gcoefs) R.gff.Bspline_PLQ = nullptr;
+    return itr;
+}
+
+// The reference's own sample_* term harness semantics (libMMFF.h:439-473):
+// h1=(1,0,0), h2=(cos a, sin a, 0); returns E and f1.y.
+void ref_sample_terms(int kind, double k, double ang0, double r1, double r2, int n, const double* angles, double* Es,
+                      double* Fs) {
+    Vec3d h1{1, 0, 0}, f1, f2;
+    Vec2d cs0;
+    cs0.fromAngle(ang0 / 2.);
+    for (int i = 0; i < n; i++) {
+        Vec3d h2{cos(angles[i]), sin(angles[i]), 0.0};
+        if (kind == 0) Es[i] = evalPiAling(h1, h2, 1. / r1, 1. / r2, k, f1, f2);
+        else if (kind == 1) Es[i] = evalAngleCos(h1, h2, 1. / r1, 1. / r2, k, cos(ang0), f1, f2);
+        else Es[i] = evalAngleCosHalf(h1, h2, 1. / r1, 1. / r2, cs0, k, f1, f2);
+        Fs[i] = f1.y;
+    }
+}
+
+// getLJQH pair energy/force along x (Forces.h:341-360); F = force on the +x atom.
+void ref_sample_pair(int n, const double* rs, const double* REQij, double* Es, double* Fs) {
+    Quat4d q{REQij[0], REQij[1], REQij[2], REQij[3]};
+    for (int i = 0; i < n; i++) {
+        Vec3d f;
+        Es[i] = getLJQH(Vec3d{rs[i], 0, 0}, f, q, 1.0);
+        Fs[i] = -f.x;
+    }
+}
+
+}  // extern "C"
