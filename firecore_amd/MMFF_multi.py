@@ -1,0 +1,553 @@
+"""ctypes surface of libMMFFmulti_lib.so with the API of pyBall/MMFF_multi.py.
+
+Same function names, argument order, defaults and buffer shapes as the
+reference module (pyBall/MMFF_multi.py:99-599), bound to this package's
+library.  Known defects of the reference module are not reproduced:
+  * setSwitches_multi follows the C argument order (MMFFmulti_lib.cpp:316),
+    the reference module's Python order (:337-338) disagrees with its C side;
+  * setTrjName binds the C signature's three arguments (libMMFF.h:53);
+  * rotate_atoms_ax and rotate_atoms are bound separately (:515-534 rebinds one);
+  * sampleNonBond returns the arrays it filled (:141-149 references Fs);
+  * scanTranslation passes len(sel) (:550-552 uses an undefined n).
+"""
+import ctypes
+from ctypes import c_bool, c_char_p, c_double, c_int, c_void_p
+
+import numpy as np
+
+from . import native
+
+c_double_p = ctypes.POINTER(c_double)
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_int_p = ctypes.POINTER(c_int)
+
+
+def _np_as(arr, atype):
+    if arr is None:
+        return None
+    return arr.ctypes.data_as(atype)
+
+
+lib = native.lib()
+array1i = np.ctypeslib.ndpointer(dtype=np.int32, ndim=1, flags="CONTIGUOUS")
+array1d = np.ctypeslib.ndpointer(dtype=np.double, ndim=1, flags="CONTIGUOUS")
+array2d = np.ctypeslib.ndpointer(dtype=np.double, ndim=2, flags="CONTIGUOUS")
+
+isInitialized = False
+glob_bMMFF = True
+nSys = 0
+
+
+def cstr(s):
+    return None if s is None else s.encode("utf8")
+
+
+# ----------------------------------------------------------- term samplers
+lib.sample_DistConstr.argtypes = [c_double, c_double, c_double, c_double, c_double, c_int, c_double_p, c_double_p,
+                                  c_double_p]
+lib.sample_DistConstr.restype = None
+
+
+def sample_DistConstr(xs, lmin=1, lmax=1, kmin=1, kmax=1, flim=1e+300, Es=None, Fs=None):
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    n = len(xs)
+    Es = np.zeros(n) if Es is None else Es
+    Fs = np.zeros(n) if Fs is None else Fs
+    lib.sample_DistConstr(lmin, lmax, kmin, kmax, flim, n, _np_as(xs, c_double_p), _np_as(Es, c_double_p),
+                          _np_as(Fs, c_double_p))
+    return Es, Fs
+
+
+def _sampler(name):
+    f = getattr(lib, name)
+    f.argtypes = [c_double, c_double, c_double, c_double, c_int, c_double_p, c_double_p, c_double_p]
+    f.restype = None
+
+    def call(angles, K=1.0, ang0=0.0, r1=1., r2=1., Es=None, Fs=None):
+        angles = np.ascontiguousarray(angles, dtype=np.float64)
+        n = len(angles)
+        Es = np.zeros(n) if Es is None else Es
+        Fs = np.zeros(n) if Fs is None else Fs
+        f(K, ang0, r1, r2, n, _np_as(angles, c_double_p), _np_as(Es, c_double_p), _np_as(Fs, c_double_p))
+        return Es, Fs
+    call.__name__ = name
+    return call
+
+
+sample_evalPiAling = _sampler("sample_evalPiAling")
+sample_evalAngleCos = _sampler("sample_evalAngleCos")
+sample_evalAngleCosHalf = _sampler("sample_evalAngleCosHalf")
+
+lib.sampleNonBond.argtypes = [c_int, array1d, array1d, array1d, c_int, array1d, array1d, c_double, c_double]
+lib.sampleNonBond.restype = None
+
+
+def sampleNonBond(rs, Es=None, fs=None, kind=1, REQi=(1.487, 0.0006808, 0.0), REQj=(1.487, 0.0006808, 0.0), K=-1.0,
+                  Rdamp=1.0):
+    rs = np.ascontiguousarray(rs, dtype=np.float64)
+    n = len(rs)
+    Es = np.zeros(n) if Es is None else Es
+    fs = np.zeros(n) if fs is None else fs
+    REQi = np.zeros(4) + np.r_[np.asarray(REQi, np.float64), np.zeros(4 - len(REQi))]
+    REQj = np.zeros(4) + np.r_[np.asarray(REQj, np.float64), np.zeros(4 - len(REQj))]
+    lib.sampleNonBond(n, rs, Es, fs, kind, REQi, REQj, K, Rdamp)
+    return Es, fs
+
+
+lib.sampleSurf.argtypes = [c_char_p, c_int, array1d, array1d, array1d, c_int, c_int, c_double, c_double, c_double,
+                           array1d, c_bool]
+lib.sampleSurf.restype = None
+
+
+def sampleSurf(name, rs, Es=None, fs=None, kind=1, atyp=0, Q=0.0, K=-1.0, Rdamp=1.0, pos0=(0., 0., 0.), bSave=False):
+    rs = np.ascontiguousarray(rs, dtype=np.float64)
+    n = len(rs)
+    Es = np.zeros(n) if Es is None else Es
+    fs = np.zeros(n) if fs is None else fs
+    lib.sampleSurf(cstr(name), n, rs, Es, fs, kind, atyp, Q, K, Rdamp, np.array(pos0, np.float64), bSave)
+    return Es, fs
+
+
+lib.sampleSurf_vecs.argtypes = [c_char_p, c_int, array2d, array1d, array2d, c_int, c_int, c_double, c_double,
+                                c_double, array1d, c_bool]
+lib.sampleSurf_vecs.restype = None
+
+
+def sampleSurf_vecs(name, poss, Es=None, fs=None, kind=1, atyp=0, Q=0.0, K=-1.0, Rdamp=1.0, pos0=(0., 0., 0.),
+                    bSave=False):
+    poss = np.ascontiguousarray(poss, dtype=np.float64)
+    n = len(poss)
+    Es = np.zeros(n) if Es is None else Es
+    fs = np.zeros((n, 3)) if fs is None else fs
+    lib.sampleSurf_vecs(cstr(name), n, poss, Es, fs, kind, atyp, Q, K, Rdamp, np.array(pos0, np.float64), bSave)
+    return Es, fs
+
+
+# ----------------------------------------------------------- buffers
+lib.printBuffNames.argtypes = []
+lib.printBuffNames.restype = None
+
+
+def printBuffNames():
+    lib.printBuffNames()
+
+
+lib.getIBuff.argtypes = [c_char_p]
+lib.getIBuff.restype = c_int_p
+lib.getBuff.argtypes = [c_char_p]
+lib.getBuff.restype = c_double_p
+lib.getfBuff.argtypes = [c_char_p]
+lib.getfBuff.restype = c_float_p
+
+
+def getIBuff(name, sh):
+    sh = sh if isinstance(sh, tuple) else (sh,)
+    return np.ctypeslib.as_array(lib.getIBuff(name.encode("utf8")), shape=sh)
+
+
+def getBuff(name, sh):
+    sh = sh if isinstance(sh, tuple) else (sh,)
+    return np.ctypeslib.as_array(lib.getBuff(name.encode("utf8")), shape=sh)
+
+
+def getfBuff(name, sh):
+    sh = sh if isinstance(sh, tuple) else (sh,)
+    return np.ctypeslib.as_array(lib.getfBuff(name.encode("utf8")), shape=sh)
+
+
+def getBuffs(NEIGH_MAX=4):
+    """Numpy views with the reference shapes (MMFF_multi.py:210-256).  The
+    gpu_* views need a uniform batch (every system the same size); for uneven
+    batches use firecore_amd.native.Batch (CSR views)."""
+    global ndims, Es, nDOFs, natoms, nnode, ncap, npi, nvecs
+    global DOFs, fDOFs, vDOFs, apos, fapos, pipos, fpipos, neighs, selection
+    global gpu_neighs, gpu_neighCell, gpu_bkNeighs, gpu_atoms, gpu_aforces, gpu_avel, gpu_constr
+    global gpu_REQs, gpu_MMpars, gpu_BLs, gpu_BKs, gpu_Ksp, gpu_Kpp, gpu_lvecs, gpu_ilvecs
+    ndims = getIBuff("ndims", (6,))
+    Es = getBuff("Es", (6,))
+    nDOFs, nnode, ncap, nvecs = int(ndims[0]), int(ndims[1]), int(ndims[2]), int(ndims[3])
+    natoms = nnode + ncap
+    npi = nnode
+    apos = getBuff("apos", (natoms, 3))
+    fapos = getBuff("fapos", (natoms, 3))
+    if glob_bMMFF:
+        DOFs = getBuff("DOFs", (nvecs, 3))
+        fDOFs = getBuff("fDOFs", (nvecs, 3))
+        vDOFs = getBuff("vDOFs", (nvecs, 3))
+        pipos = getBuff("pipos", (npi, 3))
+        fpipos = getBuff("fpipos", (npi, 3))
+        neighs = getIBuff("neighs", (natoms, NEIGH_MAX))
+        selection = getIBuff("selection", (natoms,))
+    s = native.batch_sizes()
+    if s["ntotV"] != s["nsys"] * nvecs or s["ntotA"] != s["nsys"] * natoms:
+        raise ValueError("getBuffs(): the batch is not uniform; use firecore_amd.native.Batch for CSR views")
+    ns = s["nsys"]
+    gpu_neighs = getIBuff("gpu_neighs", (ns, natoms, 4))
+    gpu_neighCell = getIBuff("gpu_neighCell", (ns, natoms, 4))
+    gpu_bkNeighs = getIBuff("gpu_bkNeighs", (ns, nvecs, 4))
+    gpu_atoms = getfBuff("gpu_atoms", (ns, nvecs, 4))
+    gpu_aforces = getfBuff("gpu_aforces", (ns, nvecs, 4))
+    gpu_avel = getfBuff("gpu_avel", (ns, nvecs, 4))
+    gpu_constr = getfBuff("gpu_constr", (ns, natoms, 4))
+    gpu_REQs = getfBuff("gpu_REQs", (ns, natoms, 4))
+    gpu_MMpars = getfBuff("gpu_MMpars", (ns, nnode, 4))
+    gpu_BLs = getfBuff("gpu_BLs", (ns, nnode, 4))
+    gpu_BKs = getfBuff("gpu_BKs", (ns, nnode, 4))
+    gpu_Ksp = getfBuff("gpu_Ksp", (ns, nnode, 4))
+    gpu_Kpp = getfBuff("gpu_Kpp", (ns, nnode, 4))
+    gpu_lvecs = getfBuff("gpu_lvecs", (ns, 3, 4))
+    gpu_ilvecs = getfBuff("gpu_ilvecs", (ns, 3, 4))
+
+
+# ----------------------------------------------------------- world
+lib.init_buffers.argtypes = []
+lib.init_buffers.restype = None
+
+
+def init_buffers():
+    return lib.init_buffers()
+
+
+lib.setVerbosity.argtypes = [c_int, c_int]
+lib.setVerbosity.restype = None
+
+
+def setVerbosity(verbosity=1, idebug=0):
+    return lib.setVerbosity(verbosity, idebug)
+
+
+lib.init.argtypes = [c_int, c_char_p, c_char_p, c_char_p, c_bool, c_bool, array1i, array1i, c_double, c_char_p,
+                     c_char_p, c_char_p, c_char_p, c_double, c_double, c_int, c_int, c_double, c_double, c_int]
+lib.init.restype = c_void_p
+
+
+def init(nSys_=10, xyz_name="input.xyz", surf_name=None, smile_name=None, sElementTypes="data/ElementTypes.dat",
+         sAtomTypes="data/AtomTypes.dat", sBondTypes="data/BondTypes.dat", sAngleTypes="data/AngleTypes.dat",
+         bMMFF=True, bEpairs=False, nPBC=(1, 1, 0), gridnPBC=(1, 1, 0), gridStep=0.1, T=-1, gamma=-1, nExplore=0,
+         nRelax=0, pos_kick=0.0, vel_kick=0.0, GridFF=5):
+    """xyz_name 'synth:natoms=N,seed=S,sigma=X' builds nSys_ replicas of a
+    synthetic molecule; periodic molecules are not supported (use nPBC=(0,0,0))."""
+    global glob_bMMFF, nSys, isInitialized
+    nSys = nSys_
+    glob_bMMFF = bMMFF
+    nPBC = np.array(nPBC, dtype=np.int32)
+    gridnPBC = np.array(gridnPBC, dtype=np.int32)
+    r = lib.init(nSys, cstr(xyz_name), cstr(surf_name), cstr(smile_name), bMMFF, bEpairs, nPBC, gridnPBC, gridStep,
+                 cstr(sElementTypes), cstr(sAtomTypes), cstr(sBondTypes), cstr(sAngleTypes), T, gamma, nExplore,
+                 nRelax, pos_kick, vel_kick, GridFF)
+    if not r:
+        raise RuntimeError("init() failed (see stderr)")
+    isInitialized = True
+    return r
+
+
+lib.insertSMILES.argtypes = [c_char_p]
+lib.insertSMILES.restype = None
+
+
+def insertSMILES(s):
+    return lib.insertSMILES(s.encode("utf8"))
+
+
+lib.setSwitches_multi.argtypes = [c_int] * 9
+lib.setSwitches_multi.restype = None
+
+
+def setSwitches_multi(CheckInvariants=0, PBC=0, NonBonded=0, MMFF=0, Angles=0, PiSigma=0, PiPiI=0, dovdW=0,
+                      bSaveToDatabase=0):
+    return lib.setSwitches_multi(CheckInvariants, PBC, NonBonded, MMFF, Angles, PiSigma, PiPiI, dovdW,
+                                 bSaveToDatabase)
+
+
+lib.checkInvariants.argtypes = [c_double, c_double, c_double]
+lib.checkInvariants.restype = c_bool
+
+
+def checkInvariants(maxVcog=1e-9, maxFcog=1e-9, maxTg=1e-1):
+    return lib.checkInvariants(maxVcog, maxFcog, maxTg)
+
+
+lib.set_opt.argtypes = [c_double, c_double, c_double, c_double, c_double, c_double, c_int, c_double, c_double]
+lib.set_opt.restype = None
+
+
+def set_opt(dt_max=0.1, dt_min=0.02, damp_max=0.2, finc=1.1, fdec=0.5, falpha=0.8, minLastNeg=5, cvf_min=-0.1,
+            cvf_max=+0.1):
+    return lib.set_opt(dt_max, dt_min, damp_max, finc, fdec, falpha, minLastNeg, cvf_min, cvf_max)
+
+
+lib.setOptLog.argtypes = [c_int, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p]
+lib.setOptLog.restype = None
+
+
+def setOptLog(n):
+    arrs = [np.zeros(n) for _ in range(5)]
+    lib.setOptLog(n, *[_np_as(a, c_double_p) for a in arrs])
+    return tuple(arrs)
+
+
+lib.setTrjName.argtypes = [c_char_p, c_int, c_int_p]
+lib.setTrjName.restype = c_bool
+
+
+def setTrjName(trj_fname_="trj.xyz", savePerNsteps=1, nPBC=(1, 1, 1)):
+    nPBC = np.array(nPBC, np.int32)
+    return lib.setTrjName(cstr(trj_fname_), savePerNsteps, _np_as(nPBC, c_int_p))
+
+
+lib.saveXYZ.argtypes = [c_char_p, c_char_p, c_int]
+lib.saveXYZ.restype = c_int
+
+
+def saveXYZ(fname, comment, imod=1):
+    return lib.saveXYZ(cstr(fname), cstr(comment), imod)
+
+
+lib.getTypeName.argtypes = [c_int, c_bool]
+lib.getTypeName.restype = c_char_p
+
+
+def getTypeName(ia, fromFF=True):
+    return lib.getTypeName(ia, fromFF).decode()
+
+
+lib.eval.argtypes = []
+lib.eval.restype = c_double
+
+
+def eval():
+    return lib.eval()
+
+
+lib.run.argtypes = [c_int, c_double, c_double, c_int, c_double_p, c_double_p, c_int]
+lib.run.restype = c_int
+
+
+def run(nstepMax=1000, dt=-1, Fconv=1e-3, ialg=2, outE=None, outF=None, iParalel=1):
+    return lib.run(nstepMax, dt, Fconv, ialg, _np_as(outE, c_double_p), _np_as(outF, c_double_p), iParalel)
+
+
+lib.MDloop.argtypes = [c_int, c_double, c_int, c_int]
+lib.MDloop.restype = None
+
+
+def MDloop(perframe=100, Ftol=-1, iParalel=3, perVF=100):
+    return lib.MDloop(perframe, Ftol, iParalel, perVF)
+
+
+# ----------------------------------------------------------- replicas
+lib.pack_system.argtypes = [c_int, c_bool, c_bool, c_bool, c_bool]
+lib.pack_system.restype = None
+
+
+def pack_system(isys, bParams=False, bForces=False, bVel=False, blvec=True):
+    return lib.pack_system(isys, bParams, bForces, bVel, blvec)
+
+
+lib.unpack_system.argtypes = [c_int, c_bool, c_bool]
+lib.unpack_system.restype = None
+
+
+def unpack_system(isys, bForces=True, bVel=True):
+    return lib.unpack_system(isys, bForces, bVel)
+
+
+lib.upload_sys.argtypes = [c_int, c_bool, c_bool, c_bool, c_bool]
+lib.upload_sys.restype = None
+
+
+def upload_sys(isys, bParams=True, bForces=False, bVel=True, blvec=True):
+    return lib.upload_sys(isys, bParams, bForces, bVel, blvec)
+
+
+lib.download_sys.argtypes = [c_int, c_bool, c_bool]
+lib.download_sys.restype = None
+
+
+def download_sys(isys, bForces=True, bVel=True):
+    return lib.download_sys(isys, bForces, bVel)
+
+
+lib.upload.argtypes = [c_bool, c_bool, c_bool, c_bool]
+lib.upload.restype = None
+
+
+def upload(bParams=True, bForces=False, bVel=True, blvec=True):
+    return lib.upload(bParams, bForces, bVel, blvec)
+
+
+lib.download.argtypes = [c_bool, c_bool]
+lib.download.restype = None
+
+
+def download(bForces=True, bVel=True):
+    return lib.download(bForces, bVel)
+
+
+# ----------------------------------------------------------- not on the path
+lib.change_lvec.argtypes = [c_double_p, c_bool, c_bool]
+lib.change_lvec.restype = None
+
+
+def change_lvec(lvec, bAdd=False, bUpdatePi=False):
+    lvec = np.array(lvec, dtype=np.float64)
+    return lib.change_lvec(_np_as(lvec, c_double_p), bAdd, bUpdatePi)
+
+
+lib.optimizeLattice_1d.argtypes = [c_double_p, c_int, c_int, c_int, c_double]
+lib.optimizeLattice_1d.restype = None
+
+
+def optimizeLattice_1d(dlvec, n1=5, n2=5, initMode=0, tol=1e-6):
+    dlvec = np.array(dlvec, dtype=np.float64)
+    return lib.optimizeLattice_1d(_np_as(dlvec, c_double_p), n1, n2, initMode, tol)
+
+
+lib.optimizeLattice_2d_multi.argtypes = [c_double_p, c_int, c_int, c_double]
+lib.optimizeLattice_2d_multi.restype = None
+
+
+def optimizeLattice_2d_multi(dlvec, nstesp=10, initMode=0, tol=1e-6):
+    dlvec = np.array(dlvec, dtype=np.float64)
+    return lib.optimizeLattice_2d_multi(_np_as(dlvec, c_double_p), nstesp, initMode, tol)
+
+
+lib.upload_pop.argtypes = [c_char_p]
+lib.upload_pop.restype = None
+
+
+def upload_pop(fname):
+    return lib.upload_pop(cstr(fname))
+
+
+lib.addDistConstrain.argtypes = [c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_double_p, c_bool]
+lib.addDistConstrain.restype = None
+
+
+def addDistConstrain(i0, i1, lmin=1, lmax=1, kmin=1, kmax=1, flim=1e+300, l=None, k=None, shift=(0., 0., 0.),
+                     bOldIndex=False):
+    if l is not None:
+        lmin = lmax = l
+    if k is not None:
+        kmin = kmax = k
+    shift = np.array(shift, np.float64)
+    return lib.addDistConstrain(i0, i1, lmin, lmax, kmin, kmax, flim, _np_as(shift, c_double_p), bOldIndex)
+
+
+lib.addAngConstrain.argtypes = [c_int, c_int, c_int, c_double, c_double]
+lib.addAngConstrain.restype = None
+
+
+def addAngConstrain(i0, i1, i2, ang0=0.0, k=1.0):
+    return lib.addAngConstrain(i0, i1, i2, ang0, k)
+
+
+def _sel(sel):
+    if sel is None:
+        return 0, None
+    sel = np.array(sel, np.int32)
+    return len(sel), sel
+
+
+lib.shift_atoms_ax.argtypes = [c_int, c_int_p, c_double_p]
+lib.shift_atoms_ax.restype = None
+
+
+def shift_atoms_ax(d, sel=None):
+    n, sel = _sel(sel)
+    d = None if d is None else np.array(d, np.float64)
+    return lib.shift_atoms_ax(n, _np_as(sel, c_int_p), _np_as(d, c_double_p))
+
+
+lib.shift_atoms.argtypes = [c_int, c_int_p, c_int, c_int, c_double]
+lib.shift_atoms.restype = None
+
+
+def shift_atoms(ia0, ia1, l, sel=None):
+    n, sel = _sel(sel)
+    return lib.shift_atoms(n, _np_as(sel, c_int_p), ia0, ia1, l)
+
+
+lib.rotate_atoms_ax.argtypes = [c_int, c_int_p, c_double_p, c_double_p, c_double]
+lib.rotate_atoms_ax.restype = None
+
+
+def rotate_atoms_ax(phi, sel=None, p0=None, ax=None):
+    n, sel = _sel(sel)
+    p0 = None if p0 is None else np.array(p0, np.float64)
+    ax = None if ax is None else np.array(ax, np.float64)
+    return lib.rotate_atoms_ax(n, _np_as(sel, c_int_p), _np_as(p0, c_double_p), _np_as(ax, c_double_p), phi)
+
+
+lib.rotate_atoms.argtypes = [c_int, c_int_p, c_int, c_int, c_int, c_double]
+lib.rotate_atoms.restype = None
+
+
+def rotate_atoms(ia0, iax0, iax1, phi, sel=None):
+    n, sel = _sel(sel)
+    return lib.rotate_atoms(n, _np_as(sel, c_int_p), ia0, iax0, iax1, phi)
+
+
+lib.scanTranslation_ax.argtypes = [c_int, c_int_p, c_double_p, c_int, c_double_p, c_char_p, c_bool]
+lib.scanTranslation_ax.restype = None
+
+
+def scanTranslation_ax(nstep, Es=None, bWriteTrj=False, sel=None, vec=None, trjName=None, bAddjustCaps=False):
+    n, sel = _sel(sel)
+    Es = np.zeros(nstep) if Es is None else Es
+    vec = None if vec is None else np.array(vec, np.float64)
+    lib.scanTranslation_ax(n, _np_as(sel, c_int_p), _np_as(vec, c_double_p), nstep, _np_as(Es, c_double_p),
+                           cstr(trjName), bAddjustCaps)
+    return Es
+
+
+lib.scanTranslation.argtypes = [c_int, c_int_p, c_int, c_int, c_double, c_int, c_double_p, c_char_p, c_bool]
+lib.scanTranslation.restype = None
+
+
+def scanTranslation(ia0, ia1, l, nstep, Es=None, sel=None, bWriteTrj=False, trjName=None, bAddjustCaps=False):
+    n, sel = _sel(sel)
+    Es = np.zeros(nstep) if Es is None else Es
+    lib.scanTranslation(n, _np_as(sel, c_int_p), ia0, ia1, l, nstep, _np_as(Es, c_double_p), cstr(trjName),
+                        bAddjustCaps)
+    return Es
+
+
+lib.scanRotation_ax.argtypes = [c_int, c_int_p, c_double_p, c_double_p, c_double, c_int, c_double_p, c_char_p]
+lib.scanRotation_ax.restype = None
+
+
+def scanRotation_ax(phi, nstep, sel=None, p0=None, ax=None, Es=None, bWriteTrj=False, trjName=None):
+    n, sel = _sel(sel)
+    Es = np.zeros(nstep) if Es is None else Es
+    p0 = None if p0 is None else np.array(p0, np.float64)
+    ax = None if ax is None else np.array(ax, np.float64)
+    lib.scanRotation_ax(n, _np_as(sel, c_int_p), _np_as(p0, c_double_p), _np_as(ax, c_double_p), phi, nstep,
+                        _np_as(Es, c_double_p), cstr(trjName))
+    return Es
+
+
+lib.scanRotation.argtypes = [c_int, c_int_p, c_int, c_int, c_int, c_double, c_int, c_double_p, c_char_p]
+lib.scanRotation.restype = None
+
+
+def scanRotation(ia0, iax0, iax1, phi, nstep, sel=None, Es=None, bWriteTrj=False, _0=0, trjName=None):
+    n, sel = _sel(sel)
+    Es = np.zeros(nstep) if Es is None else Es
+    lib.scanRotation(n, _np_as(sel, c_int_p), ia0 - _0, iax0 - _0, iax1 - _0, phi, nstep, _np_as(Es, c_double_p),
+                     cstr(trjName))
+    return Es
+
+
+lib.scan.argtypes = [c_int, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p, c_bool, c_bool, c_int,
+                     c_double, c_double, c_double]
+lib.scan.restype = None
+
+
+def scan(poss, rots=None, Es=None, aforces=None, aposs=None, bF=False, bP=False, omp=False, bRelax=False,
+         niter_max=10000, dt=0.05, Fconv=1e-5, Flim=100.0):
+    nconf = len(poss)
+    Es = np.zeros(nconf) if Es is None else Es
+    lib.scan(nconf, _np_as(poss, c_double_p), _np_as(rots, c_double_p), _np_as(Es, c_double_p),
+             _np_as(aforces, c_double_p), _np_as(aposs, c_double_p), omp, bRelax, niter_max, dt, Fconv, Flim)
+    return Es, aforces, aposs

@@ -686,6 +686,7 @@ int init_systems(int nSys, const int* natoms, const int* nnode, const double* ap
         in += M;
         iv += V;
     }
+    W.bGridFF = false;  // a new batch has no substrate until one is loaded
     if (!W.set_systems(std::move(ms))) return -1;
     if (!finish_init(W)) return -1;
     return W.ntotA;
@@ -720,64 +721,9 @@ int init_workload(int kind, int nSys, long long seed0, int nmin, int nmax, doubl
         std::vector<std::thread> th;
         for (int t = 0; t < nth; t++)
             th.emplace_back([&, t]() {
-                for (int i = t; i < nSys; i += nth) {
-                    const uint64_t sd = (uint64_t)(seed0 + i);
-                    fc::Rng rng(sd * 2654435761ull + 17);
-                    fc::MolSys m;
-                    if (kind == 1) {
-                        int n = nmin + rng.randint(nmax - nmin + 1);
-                        if (!fc::gen_molecule(n, sd, m)) { fail = true; continue; }
-                    } else {
-                        fc::MolSys a, b;
-                        int na = nmin + rng.randint(nmax - nmin + 1);
-                        int nb = nmin + rng.randint(nmax - nmin + 1);
-                        if (!fc::gen_molecule(na, sd * 2 + 0x1000000ull, a) ||
-                            !fc::gen_molecule(nb, sd * 2 + 0x1000001ull, b)) { fail = true; continue; }
-                        fc::place_molecule(a, sd * 3 + 1, 0.0, 0.0, 0.0);
-                        fc::place_molecule(b, sd * 3 + 2, 0.0, 0.0, 0.0);
-                        // separate the centroids by 4-8 A along a random direction,
-                        // pushed apart until the closest pair is >= 2.5 A
-                        double dir[3] = {rng.normal(), rng.normal(), rng.normal()};
-                        double nd = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-                        for (double& x : dir) x /= nd;
-                        double sep = rng.uniform(4.0, 8.0);
-                        auto centroid = [](const fc::MolSys& x, double* c) {
-                            c[0] = c[1] = c[2] = 0;
-                            for (int k = 0; k < x.natoms; k++)
-                                for (int q = 0; q < 3; q++) c[q] += x.apos[k * 3 + q] / x.natoms;
-                        };
-                        double ca[3], cb[3];
-                        centroid(a, ca);
-                        centroid(b, cb);
-                        for (int it = 0; it < 200; it++) {
-                            double shift[3];
-                            for (int q = 0; q < 3; q++) shift[q] = ca[q] + dir[q] * sep - cb[q];
-                            double dmin = 1e9;
-                            for (int x = 0; x < a.natoms; x++)
-                                for (int y = 0; y < b.natoms; y++) {
-                                    double d2 = 0;
-                                    for (int q = 0; q < 3; q++) {
-                                        double d = a.apos[x * 3 + q] - (b.apos[y * 3 + q] + shift[q]);
-                                        d2 += d * d;
-                                    }
-                                    dmin = std::min(dmin, d2);
-                                }
-                            if (std::sqrt(dmin) >= 2.5) {
-                                for (int y = 0; y < b.natoms; y++)
-                                    for (int q = 0; q < 3; q++) b.apos[y * 3 + q] += shift[q];
-                                break;
-                            }
-                            sep += 0.25;
-                        }
-                        fc::merge_molecules(a, b, m);
-                    }
-                    fc::perturb_positions(m, sd + 7777777ull, sigma);
-                    if (surf) {
-                        fc::place_molecule(m, sd + 31337ull, g.pos0[2] + 3.0 + rng.uniform(-0.3, 0.3),
-                                           g.pos0[0] + rng.uniform(0, ax), g.pos0[1] + rng.uniform(0, by));
-                    }
-                    ms[i] = std::move(m);
-                }
+                for (int i = t; i < nSys; i += nth)
+                    if (!fc::make_workload_system(kind, i, seed0, nmin, nmax, sigma, surf, g.pos0, ax, by, ms[i]))
+                        fail = true;
             });
         for (auto& x : th) x.join();
     }
@@ -804,6 +750,33 @@ int relax(int nstepMax, double Fconv, int path, bool restore_start, bool bDownlo
 
 double get_last_ms() { return world().last_ms; }
 
+// Size classes of the persistent kernel (one launch each): per class
+// {block, nsys, nmax, mmax, ngroups, lds_bytes} into info6[6*k] and the device
+// time of its launch in the last relax into ms[k].  Returns the class count.
+int get_class_info(int* info6, double* ms, int maxc) {
+    World& W = world();
+    int n = std::min((int)W.classes.size(), maxc);
+    for (int c = 0; c < n; c++) {
+        const fcw::SizeClass& C = W.classes[c];
+        info6[6 * c + 0] = C.block;
+        info6[6 * c + 1] = (int)C.order.size();
+        info6[6 * c + 2] = C.nmax;
+        info6[6 * c + 3] = C.mmax;
+        info6[6 * c + 4] = C.ngroups;
+        info6[6 * c + 5] = (int)C.lds;
+        ms[c] = C.last_ms;
+    }
+    return n;
+}
+// System indices of class c in queue order (largest first); returns count.
+int get_class_order(int c, int* out, int maxn) {
+    World& W = world();
+    if (c < 0 || c >= (int)W.classes.size()) return 0;
+    int n = std::min((int)W.classes[c].order.size(), maxn);
+    for (int i = 0; i < n; i++) out[i] = W.classes[c].order[i];
+    return n;
+}
+
 // Evaluate forces and energies of every system at the current positions
 // (split kernels, no move); results in gpu_aforces / sys_E after download.
 int eval_forces() {
@@ -813,6 +786,11 @@ int eval_forces() {
 }
 
 void set_conv_mode(int mode) { world().conv_mode = mode; }
+
+// Per-vec force clamp for the FIRE dynamics (the reference GPU update kernel
+// clamps |f| to 10 eV/A, relax_multi.cl:980-983); 0 disables it (the CPU
+// path's behaviour).  Convergence is always tested on the unclamped force.
+void set_force_limit(double Flimit) { world().Flimit = (float)Flimit; }
 
 // nSys, ntotA, ntotM, ntotV, maxN, maxM, grid nx, ny, nz
 void get_batch_sizes(int* out) {
@@ -858,6 +836,29 @@ int gen_molecule_arrays(int natoms, long long seed, double sigma, long long psee
     if (Kpp) std::memcpy(Kpp, m.Kpp.data(), m.Kpp.size() * 8);
     if (elem) std::memcpy(elem, m.elem.data(), m.elem.size() * 4);
     return m.nnode;
+}
+// One system of init_workload (kind 1 or 2), generated on the host only.
+// pos0/cell: grid origin and lateral cell of the substrate (NULL: vacuum).
+int gen_workload_arrays(int kind, int i, long long seed0, int nmin, int nmax, double sigma, const double* pos0,
+                        const double* cell, int* natoms_out, int* nnode_out, double* apos, double* REQs, int* neighs,
+                        double* apars, double* bLs, double* bKs, double* Ksp, double* Kpp, int* elem) {
+    fc::MolSys m;
+    const double z[3] = {0, 0, 0};
+    if (!fc::make_workload_system(kind, i, seed0, nmin, nmax, sigma, pos0 != nullptr, pos0 ? pos0 : z,
+                                  cell ? cell[0] : 0, cell ? cell[1] : 0, m))
+        return -1;
+    *natoms_out = m.natoms;
+    *nnode_out = m.nnode;
+    if (apos) std::memcpy(apos, m.apos.data(), m.apos.size() * 8);
+    if (REQs) std::memcpy(REQs, m.REQs.data(), m.REQs.size() * 8);
+    if (neighs) std::memcpy(neighs, m.neighs.data(), m.neighs.size() * 4);
+    if (apars) std::memcpy(apars, m.apars.data(), m.apars.size() * 8);
+    if (bLs) std::memcpy(bLs, m.bLs.data(), m.bLs.size() * 8);
+    if (bKs) std::memcpy(bKs, m.bKs.data(), m.bKs.size() * 8);
+    if (Ksp) std::memcpy(Ksp, m.Ksp.data(), m.Ksp.size() * 8);
+    if (Kpp) std::memcpy(Kpp, m.Kpp.data(), m.Kpp.size() * 8);
+    if (elem) std::memcpy(elem, m.elem.data(), m.elem.size() * 4);
+    return m.natoms;
 }
 int make_back_neighs(int natoms, int nnode, const int* neighs, int* bkneighs) {
     return fc::make_back_neighs(natoms, nnode, neighs, bkneighs) ? 0 : -1;

@@ -68,6 +68,9 @@ void place_molecule(MolSys& m, uint64_t seed, double ztarget, double cx, double 
 void flip_pis(MolSys& m, const double ax[3]);
 // Pair of molecules in one system (config 4): concatenated topology.
 void merge_molecules(const MolSys& a, const MolSys& b, MolSys& out);
+// System i of a synthetic workload (kind 1: single molecules, kind 2: pairs).
+bool make_workload_system(int kind, int i, long long seed0, int nmin, int nmax, double sigma, bool surf,
+                          const double* pos0, double ax, double by, MolSys& m);
 
 // ---------------- topology (topology.cpp) ----------------
 // makeBackNeighs (MMFFsp3_loc.h:1335-1358): bk[j] lists ia*4+k for every node

@@ -61,6 +61,7 @@ struct RunParams {
     GridDev grid;
     FireParams fire;
     float Fconv2;
+    float Flimit;  // >0: clamp |f| per vec for the dynamics (relax_multi.cl:980-983); 0: off
     int conv_mode;
     int nstep_max;
 };

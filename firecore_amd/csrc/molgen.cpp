@@ -516,4 +516,66 @@ void merge_molecules(const MolSys& a, const MolSys& b, MolSys& out) {
     out.Kpp = cat(a.Kpp, b.Kpp);
 }
 
+// System i of a synthetic workload (see init_workload in capi.cpp).
+//  kind 1: one molecule, natoms ~ U[nmin,nmax], seed seed0+i
+//  kind 2: two molecules ~ U[nmin,nmax] each, random orientations, centroids
+//          4-8 A apart (pushed apart until the closest pair is >= 2.5 A)
+// Every system gets N(0,sigma) noise; with a substrate it is rotated at random
+// and placed with its lowest atom 3.0 +- 0.3 A above the grid origin plane and
+// its centroid at a random lateral point of the cell.
+bool make_workload_system(int kind, int i, long long seed0, int nmin, int nmax, double sigma, bool surf,
+                          const double* pos0, double ax, double by, MolSys& m) {
+    const uint64_t sd = (uint64_t)(seed0 + i);
+    Rng rng(sd * 2654435761ull + 17);
+    if (kind == 1) {
+        int n = nmin + rng.randint(nmax - nmin + 1);
+        if (!gen_molecule(n, sd, m)) return false;
+    } else {
+        MolSys a, b;
+        int na = nmin + rng.randint(nmax - nmin + 1);
+        int nb = nmin + rng.randint(nmax - nmin + 1);
+        if (!gen_molecule(na, sd * 2 + 0x1000000ull, a) || !gen_molecule(nb, sd * 2 + 0x1000001ull, b)) return false;
+        place_molecule(a, sd * 3 + 1, 0.0, 0.0, 0.0);
+        place_molecule(b, sd * 3 + 2, 0.0, 0.0, 0.0);
+        double dir[3] = {rng.normal(), rng.normal(), rng.normal()};
+        double nd = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        for (double& x : dir) x /= nd;
+        double sep = rng.uniform(4.0, 8.0);
+        auto centroid = [](const MolSys& x, double* c) {
+            c[0] = c[1] = c[2] = 0;
+            for (int k = 0; k < x.natoms; k++)
+                for (int q = 0; q < 3; q++) c[q] += x.apos[k * 3 + q] / x.natoms;
+        };
+        double ca[3], cb[3];
+        centroid(a, ca);
+        centroid(b, cb);
+        for (int it = 0; it < 200; it++) {
+            double shift[3];
+            for (int q = 0; q < 3; q++) shift[q] = ca[q] + dir[q] * sep - cb[q];
+            double dmin = 1e9;
+            for (int x = 0; x < a.natoms; x++)
+                for (int y = 0; y < b.natoms; y++) {
+                    double d2 = 0;
+                    for (int q = 0; q < 3; q++) {
+                        double d = a.apos[x * 3 + q] - (b.apos[y * 3 + q] + shift[q]);
+                        d2 += d * d;
+                    }
+                    dmin = std::min(dmin, d2);
+                }
+            if (std::sqrt(dmin) >= 2.5 || it == 199) {
+                for (int y = 0; y < b.natoms; y++)
+                    for (int q = 0; q < 3; q++) b.apos[y * 3 + q] += shift[q];
+                break;
+            }
+            sep += 0.25;
+        }
+        merge_molecules(a, b, m);
+    }
+    perturb_positions(m, sd + 7777777ull, sigma);
+    if (surf)
+        place_molecule(m, sd + 31337ull, pos0[2] + 3.0 + rng.uniform(-0.3, 0.3), pos0[0] + rng.uniform(0, ax),
+                       pos0[1] + rng.uniform(0, by));
+    return true;
+}
+
 }  // namespace fc

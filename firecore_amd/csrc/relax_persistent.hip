@@ -224,12 +224,13 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
                         const V3 p = xyz(s_pos[v]);
                         f -= p * dot(p, f);
                     }
+                    const float f2 = dot(f, f);
+                    fm2 = fmaxf(fm2, f2);  // convergence always on the exact force
+                    if (P.Flimit > 0.f && f2 > P.Flimit * P.Flimit) f = f * (P.Flimit / sqrtf(f2));
                     fv[r] = f;
                     vf += dot(vel[r], f);
                     vv += dot(vel[r], vel[r]);
-                    const float f2 = dot(f, f);
-                    ff += f2;
-                    fm2 = fmaxf(fm2, f2);
+                    ff += dot(f, f);
                 }
             }
             vf = wave_sum(vf);

@@ -178,13 +178,17 @@ __global__ void __launch_bounds__(256) k_update(BatchDev B, RunParams P, int do_
             const V3 p = xyz(B.atoms[I.v0 + v]);
             f -= p * dot(p, f);
         }
+        const float f2 = dot(f, f);
+        fm2 = fmaxf(fm2, f2);  // convergence always on the exact force
+        B.aforce[I.v0 + v] = make_float4(f.x, f.y, f.z, 0.f);
+        if (do_move && P.Flimit > 0.f && f2 > P.Flimit * P.Flimit) {
+            f = f * (P.Flimit / sqrtf(f2));
+            B.aforce[I.v0 + v] = make_float4(f.x, f.y, f.z, 0.f);
+        }
         const V3 w = xyz(B.avel[I.v0 + v]);
         vf += dot(w, f);
         vv += dot(w, w);
-        const float f2 = dot(f, f);
-        ff += f2;
-        fm2 = fmaxf(fm2, f2);
-        B.aforce[I.v0 + v] = make_float4(f.x, f.y, f.z, 0.f);
+        ff += dot(f, f);
     }
     for (int k = threadIdx.x; k < M; k += blockDim.x) e += (double)B.ebond[I.n0 + k];
     vf = blk_sum(vf, sh);

@@ -380,6 +380,7 @@ fcd::RunParams World::run_params(double Fconv, int nstep_max) {
     P.fire = fp;
     P.Fconv2 = (float)(Fconv * Fconv);
     P.conv_mode = conv_mode;
+    P.Flimit = Flimit;
     P.nstep_max = nstep_max;
     return P;
 }
@@ -458,19 +459,22 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     hipEventRecord(e0, stream);
-    std::vector<hipEvent_t> done;
+    std::vector<hipEvent_t> starts, done;
     int off = 0;
     for (size_t c = 0; c < classes.size(); c++) {
         SizeClass& C = classes[c];
         hipStream_t st = class_streams[c % class_streams.size()];
         hipStreamWaitEvent(st, e0, 0);
+        hipEvent_t es, ev;
+        hipEventCreate(&es);
+        hipEventCreate(&ev);
+        hipEventRecord(es, st);
         hipError_t err = fc_launch_relax_persistent(C.block, C.ngroups, C.lds, st, &B, &P, d_order.p + off,
                                                     (int)C.order.size(), d_queue.p + c, C.nmax, C.mmax, C.wmax);
         if (!hip_ok(err, "relax_persistent launch")) return -1;
-        hipEvent_t ev;
-        hipEventCreate(&ev);
         hipEventRecord(ev, st);
         hipStreamWaitEvent(stream, ev, 0);
+        starts.push_back(es);
         done.push_back(ev);
         off += (int)C.order.size();
     }
@@ -479,6 +483,12 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
     float ms = 0;
     hipEventElapsedTime(&ms, e0, e1);
     last_ms = ms;
+    for (size_t c = 0; c < classes.size(); c++) {
+        float cm = 0;
+        hipEventElapsedTime(&cm, starts[c], done[c]);
+        classes[c].last_ms = cm;
+        hipEventDestroy(starts[c]);
+    }
     for (auto ev : done) hipEventDestroy(ev);
     hipEventDestroy(e0);
     hipEventDestroy(e1);

@@ -31,6 +31,7 @@ struct SizeClass {
     double work = 0;         // sum of N^2
     int ngroups = 0;
     size_t lds = 0;
+    float last_ms = 0;  // device time of this class's launch in the last relax
 };
 
 struct World {
@@ -42,6 +43,7 @@ struct World {
     // ---- FIRE (MolWorld_sp3 opt after initOpt(dt_default=0.05), DynamicOpt.h:47-95)
     fcd::FireParams fp{0.05f, 0.005f, 0.1f, 1.1f, 0.5f, 0.8f, 0.0f, 3};
     int conv_mode = fcd::CONV_FMAX;
+    float Flimit = 0.f;  // per-vec force clamp for the dynamics (0 = off)
 
     // ---- systems (host, double) and the template system (system 0)
     std::vector<fc::MolSys> mols;

@@ -97,6 +97,39 @@ def gen_molecule(natoms, seed, sigma=0.0, pseed=0):
     return d
 
 
+def gen_workload_system(kind, i, seed0, nmin, nmax, sigma=0.1, grid=None):
+    """System i of init_workload(kind, ...) generated on the host (same arrays
+    the device batch receives, in double precision)."""
+    L = lib()
+    if not hasattr(L, "_gw"):
+        L.gen_workload_arrays.argtypes = [c_int, c_int, c_longlong, c_int, c_int, c_double, c_double_p, c_double_p,
+                                          c_int_p, c_int_p] + [c_void_p] * 9
+        L.gen_workload_arrays.restype = c_int
+        L._gw = True
+    nmx = nmax * 2 + 8
+    N, M = c_int(0), c_int(0)
+    apos = np.zeros((2 * nmx, 3))
+    REQs = np.zeros((nmx, 4))
+    neighs = np.zeros((nmx, 4), np.int32)
+    pars = [np.zeros((nmx, 4)) for _ in range(5)]
+    elem = np.zeros(nmx, np.int32)
+    pos0 = cell = None
+    if grid is not None:
+        pos0 = np.ascontiguousarray(grid["pos0"], np.float64)
+        cell = np.array([grid["n"][0] * grid["step"][0], grid["n"][1] * grid["step"][1]])
+    r = L.gen_workload_arrays(kind, i, seed0, nmin, nmax, sigma, _p(pos0), _p(cell), ctypes.byref(N),
+                              ctypes.byref(M), apos.ctypes.data, REQs.ctypes.data, neighs.ctypes.data,
+                              *[p.ctypes.data for p in pars], elem.ctypes.data)
+    if r < 0:
+        raise ValueError("cannot generate workload system")
+    n, m = N.value, M.value
+    d = dict(natoms=n, nnode=m, apos=np.ascontiguousarray(apos[:n + m]), REQs=np.ascontiguousarray(REQs[:n]),
+             neighs=np.ascontiguousarray(neighs[:n]), elem=np.ascontiguousarray(elem[:n]))
+    for k, p in zip(("apars", "bLs", "bKs", "Ksp", "Kpp"), pars):
+        d[k] = np.ascontiguousarray(p[:m])
+    return d
+
+
 def make_back_neighs(natoms, nnode, neighs):
     neighs = np.ascontiguousarray(neighs, np.int32)
     bk = np.zeros((natoms, 4), np.int32)
@@ -249,6 +282,23 @@ def last_ms():
     return lib().get_last_ms()
 
 
+def class_info():
+    """Persistent-kernel size classes of the current batch with the device
+    time (ms) of each class launch in the last relax."""
+    L = lib()
+    info = np.zeros(6 * 8, np.int32)
+    ms = np.zeros(8)
+    L.get_class_info.restype = c_int
+    n = L.get_class_info(_p(info, c_int_p), _p(ms), 8)
+    out = []
+    for c in range(n):
+        b, ns, nmax, mmax, ng, lds = (int(x) for x in info[6 * c:6 * c + 6])
+        order = np.zeros(ns, np.int32)
+        L.get_class_order(c, _p(order, c_int_p), ns)
+        out.append(dict(block=b, nsys=ns, nmax=nmax, mmax=mmax, ngroups=ng, lds=lds, ms=float(ms[c]), order=order))
+    return out
+
+
 def eval_forces():
     if lib().eval_forces() != 0:
         raise RuntimeError("eval_forces failed")
@@ -256,6 +306,12 @@ def eval_forces():
 
 def set_conv_mode(mode):
     lib().set_conv_mode(mode)
+
+
+def set_force_limit(Flimit):
+    L = lib()
+    L.set_force_limit.argtypes = [c_double]
+    L.set_force_limit(Flimit)
 
 
 def set_opt(dt_max=0.05, dt_min=0.005, damp_max=0.1, finc=1.1, fdec=0.5, falpha=0.8, minLastNeg=3,

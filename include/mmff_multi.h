@@ -1,0 +1,214 @@
+/* libMMFFmulti_lib.so -- C-ABI of the MI355X multi-system MMFF relaxation library.
+ *
+ * Drop-in for the reference's libMMFFmulti_lib.so (ProkopHapala/FireCore,
+ * built from cpp/libs_OCL/MMFFmulti_lib.cpp, which also compiles in
+ * cpp/common/molecular/libMMFF.h and cpp/common/libUtils.h) as bound by
+ * pyBall/MMFF_multi.py.  Every symbol that module binds exists here with the
+ * same C signature; each declaration cites the reference function it replaces.
+ *
+ * Differences from the reference's behaviour are stated per function.  Common
+ * to all: errors print to stderr and return a failure value (NULL / -1 / false)
+ * instead of exit(0); the process-wide state is one global world (not
+ * reentrant), as in the reference (MMFFmulti_lib.cpp:14).
+ *
+ * Additive entry points (uneven CSR batches, synthetic workloads, explicit
+ * device relaxation) follow the reference ones at the end of the file.
+ */
+#ifndef FIRECORE_AMD_MMFF_MULTI_H
+#define FIRECORE_AMD_MMFF_MULTI_H
+
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================= libUtils.h */
+/* libUtils.h:16  print the names of all registered buffers */
+void printBuffNames(void);
+/* libUtils.h:23  borrowed double buffer by name ("apos","fapos","DOFs","fDOFs",
+ * "vDOFs","pipos","fpipos","Es", additive: "sys_E"); NULL if unknown */
+double* getBuff(const char* name);
+/* libUtils.h:33  borrowed float buffer ("gpu_atoms","gpu_aforces","gpu_avel",
+ * "gpu_constr","gpu_REQs","gpu_MMpars","gpu_BLs","gpu_BKs","gpu_Ksp","gpu_Kpp",
+ * "gpu_lvecs","gpu_ilvecs","gpu_pbcshifts", additive: "gpu_PLQs","sys_Fmax") */
+float* getfBuff(const char* name);
+/* libUtils.h:43  borrowed int buffer ("neighs","gpu_neighs","gpu_neighCell",
+ * "gpu_bkNeighs","ndims","selection", additive: "sys_info","sys_nsteps",
+ * "sys_status","gpu_elem") */
+int* getIBuff(const char* name);
+
+/* ================================================= MMFFmulti_lib.cpp */
+/* MMFFmulti_lib.cpp:23-71  (re)register the named host buffers */
+void init_buffers(void);
+/* MMFFmulti_lib.cpp:75-114  Build nSys replicas and upload them.
+ * xyz_name: "synth:natoms=N,seed=S,sigma=X" -- template molecule from the
+ *   synthetic generator, replica i perturbed with seed S+1+i.  (The reference
+ *   types a molecule from an .xyz file with MMFFBuilder; that builder is not
+ *   part of this library and such names return NULL.)
+ * surf_name + GridFF>0: substrate "<surf_name>.xyz" (lvs line) or
+ *   "synth:NaCl"; coefficients from data/<surface>/Bspline_PLQd.npy, computed
+ *   and saved when missing (GridFF::tryLoad_new, GridFF.h:1787-1866).
+ * nPBC must be 0 (non-periodic molecules); type-file and global-optimisation
+ * arguments are accepted and ignored.  Returns the world pointer or NULL. */
+void* init(int nSys, char* xyz_name, char* surf_name, char* smile_name, bool bMMFF, bool bEpairs, int* nPBC,
+           int* grid_nPBC, double gridStep, char* sElementTypes, char* sAtomTypes, char* sBondTypes,
+           char* sAngleTypes, double T, double gamma, int nExplore, int nRelax, double pos_kick, double vel_kick,
+           int GridFF);
+/* MMFFmulti_lib.cpp:116-132  Relax every system to its own Fmax < Fconv (max
+ * |f| over atoms and pi-vectors) on the device; iParalel==2 uses the split
+ * per-phase kernels, every other value the persistent kernel.  dt and ialg
+ * are ignored (as in the reference).  outE/outF (nSys doubles each, may be
+ * NULL) receive per-system energy and Fmax.  Downloads atoms/forces/vel.
+ * Returns the largest step count of any system, -1 on error. */
+int run(int nstepMax, double dt, double Fconv, int ialg, double* outE, double* outF, int iParalel);
+/* MMFFmulti_lib.cpp:134-139  run(perframe, Ftol) */
+void MDloop(int perframe, double Ftol, int iParalel, int perVF);
+/* MMFFmulti_lib.cpp:141-166  FIRE parameters (cvf_min/cvf_max unused, as in the
+ * FIRE path of the reference) */
+void set_opt(double dt_max, double dt_min, double damp_max, double finc, double fdec, double falpha, int minLastNeg,
+             double cvf_min, double cvf_max);
+/* MMFFmulti_lib.cpp:168-208  GridFF probe samples along z at (pos0.x,pos0.y);
+ * kinds >= 10 sample this library's B-spline grid with a probe of element
+ * atyp (0=C,1=N,2=O,3=H) and charge Q; other kinds are not supported */
+void sampleSurf(char* name, int n, double* rs, double* Es, double* fs, int kind, int atyp, double Q, double K,
+                double RQ, double* pos0_, bool bSave);
+/* MMFFmulti_lib.cpp:210-250  same at arbitrary points poss (n x 3), forces n x 3 */
+void sampleSurf_vecs(char* name, int n, double* poss_, double* Es, double* fs_, int kind, int atyp, double Q,
+                     double K, double RQ, double* pos0_, bool bSave);
+/* MMFFmulti_lib.cpp:253-258  host system isys -> packed float mirrors */
+void pack_system(int isys, bool bParams, bool bForces, bool bVel, bool blvec);
+/* MMFFmulti_lib.cpp:256-258  packed float mirrors -> host system isys */
+void unpack_system(int isys, bool bForces, bool bVel);
+/* MMFFmulti_lib.cpp:260-262  mirrors -> device (whole batch) */
+void upload_sys(int isys, bool bParams, bool bForces, bool bVel, bool blvec);
+/* MMFFmulti_lib.cpp:263-265  device -> mirrors (whole batch) */
+void download_sys(int isys, bool bForces, bool bVel);
+/* MMFFmulti_lib.cpp:267-269 */
+void upload(bool bParams, bool bForces, bool bVel, bool blvec);
+/* MMFFmulti_lib.cpp:270-272 */
+void download(bool bForces, bool bVel);
+/* MMFFmulti_lib.cpp:275-280  not supported (periodic molecules): no-op */
+void change_lvec(double* lvec, bool bAdd, bool bUpdatePi);
+/* MMFFmulti_lib.cpp:290-293  not supported: no-op */
+void optimizeLattice_1d(double* dlvec, int n1, int n2, int initMode, double tol);
+/* MMFFmulti_lib.cpp:295-305  not supported: no-op */
+void optimizeLattice_2d_multi(double* dlvec, int nstesp, int initMode, double tol);
+/* MMFFmulti_lib.cpp:282-288  not supported: no-op */
+void upload_pop(const char* fname);
+/* MMFFmulti_lib.cpp:305-314  not supported (screening refill): no-op */
+void scan(int nconf, double* poss, double* rots, double* Es, double* aforces, double* aposs, bool omp, bool bRelax,
+          int niter_max, double dt, double Fconv, double Flim);
+/* MMFFmulti_lib.cpp:316-329  tri-state switches (>0 on, <0 off, 0 keep);
+ * bSubtractAngleNonBond follows NonBonded */
+void setSwitches_multi(int CheckInvariants, int PBC, int NonBonded, int MMFF, int Angles, int PiSigma, int PiPiI,
+                       int dovdW, int bSaveToDatabase);
+
+/* ========================================================= libMMFF.h */
+/* libMMFF.h:15 */
+void setVerbosity(int verbosity_, int idebug_);
+/* libMMFF.h:25  not supported: no-op */
+void insertSMILES(char* s);
+/* libMMFF.h:52  net velocity, net force and torque of system 0 */
+bool checkInvariants(double maxVcog, double maxFcog, double maxTg);
+/* libMMFF.h:623  not supported: no-op */
+void setOptLog(int n, double* cos, double* f, double* v, double* dt, double* damp);
+/* libMMFF.h:53  not supported: returns false */
+bool setTrjName(const char* trj_fname_, int savePerNsteps_, int* nPBC);
+/* libMMFF.h:55  all systems as concatenated xyz frames; returns #frames */
+int saveXYZ(const char* fname, const char* comment, int imod);
+/* libMMFF.h:112  element symbol of template atom ia */
+const char* getTypeName(int ia, bool fromFF);
+/* libMMFF.h (eval)  forces/energies of the batch on the device; returns E of system 0 */
+double eval(void);
+/* libMMFF.h:41  constraints not supported: no-op */
+void addDistConstrain(int i0, int i1, double lmin, double lmax, double kmin, double kmax, double flim, double* shift,
+                      bool bOldIndex);
+/* libMMFF.h:46  constraints not supported: no-op */
+void addAngConstrain(int i0, int i1, int i2, double ang0, double k);
+/* libMMFF.h:122  shift selected atoms of system 0 */
+void shift_atoms_ax(int n, int* selection, double* d);
+/* libMMFF.h:124 */
+void shift_atoms(int n, int* selection, int ia0, int ia1, double l);
+/* libMMFF.h:123  rotate selected atoms of system 0 about (p0, ax) */
+void rotate_atoms_ax(int n, int* selection, double* p0, double* ax, double phi);
+/* libMMFF.h:125 */
+void rotate_atoms(int n, int* selection, int ia0, int iax0, int iax1, double phi);
+/* libMMFF.h:134-150  scans not supported: no-op */
+void scanTranslation_ax(int n, int* selection, double* vec, int nstep, double* Es, const char* trjName,
+                        bool bAddjustCaps);
+void scanTranslation(int n, int* selection, int ia0, int ia1, double l, int nstep, double* Es, const char* trjName,
+                     bool bAddjustCaps);
+void scanRotation_ax(int n, int* selection, double* p0, double* ax, double phi, int nstep, double* Es,
+                     const char* trjName);
+void scanRotation(int n, int* selection, int ia0, int iax0, int iax1, double phi, int nstep, double* Es,
+                  const char* trjName);
+/* libMMFF.h:426  constraints not supported: zeros */
+void sample_DistConstr(double lmin, double lmax, double kmin, double kmax, double flim, int n, double* xs, double* Es,
+                       double* Fs);
+/* libMMFF.h:439  pi-pi alignment term sampled by the device (FP32) */
+void sample_evalPiAling(double k, double ang0, double r1, double r2, int n, double* angles, double* Es, double* Fs);
+/* libMMFF.h:451  pi-sigma term sampled by the device (FP32) */
+void sample_evalAngleCos(double k, double ang0, double r1, double r2, int n, double* angles, double* Es, double* Fs);
+/* libMMFF.h:463  sigma-sigma cos(theta/2) term sampled by the device (FP32) */
+void sample_evalAngleCosHalf(double k, double ang0, double r1, double r2, int n, double* angles, double* Es,
+                             double* Fs);
+/* libMMFF.h:475  kind 2: this library's getLJQH pair term on the device
+ * (REQij mixed from REQi/REQj); other kinds not supported (zeros) */
+void sampleNonBond(int n, double* rs, double* Es, double* fs, int kind, double* REQi_, double* REQj_, double K,
+                   double Rdamp);
+
+/* ============================================= additive (no reference) */
+/* Uneven batch from CSR-packed host arrays: natoms/nnode per system; rows of
+ * apos ((N+M) x 3), REQs/neighs (N x 4), apars..Kpp (M x 4), elem (N, may be
+ * NULL) concatenated in system order.  Returns total atoms or -1. */
+int init_systems(int nSys, const int* natoms, const int* nnode, const double* apos, const double* REQs,
+                 const int* neighs, const double* apars, const double* bLs, const double* bKs, const double* Ksp,
+                 const double* Kpp, const int* elem);
+/* Synthetic workloads: kind 0 = replicas of one nmin-atom molecule, kind 1 =
+ * independent molecules with N ~ U[nmin,nmax], kind 2 = molecule pairs;
+ * surf_name (NULL = vacuum) places systems 3.0+-0.3 A above the substrate. */
+int init_workload(int kind, int nSys, long long seed0, int nmin, int nmax, double sigma, const char* surf_name,
+                  double gridStep);
+/* System i of init_workload (kinds 1,2) generated on the host only. */
+int gen_workload_arrays(int kind, int i, long long seed0, int nmin, int nmax, double sigma, const double* pos0,
+                        const double* cell, int* natoms_out, int* nnode_out, double* apos, double* REQs, int* neighs,
+                        double* apars, double* bLs, double* bKs, double* Ksp, double* Kpp, int* elem);
+/* Relax the batch on the device: path 0 persistent kernel, 1 split kernels;
+ * restore_start restarts from the last uploaded positions; bDownload copies
+ * the state back to the host mirrors.  Returns max steps, -1 on error. */
+int relax(int nstepMax, double Fconv, int path, bool restore_start, bool bDownload);
+/* Device time of the last relax (HIP events), ms. */
+double get_last_ms(void);
+/* Persistent-kernel size classes: {block, nsys, nmax, mmax, ngroups, lds}
+ * per class and each launch's device time (ms) in the last relax. */
+int get_class_info(int* info6, double* ms, int maxc);
+/* System indices of size class c in queue order. */
+int get_class_order(int c, int* out, int maxn);
+/* Forces and energies of all systems at the current positions (no move). */
+int eval_forces(void);
+/* Convergence criterion: 0 = max |f| < Fconv, 1 = sum f^2 < Fconv^2. */
+void set_conv_mode(int mode);
+/* Per-vec force clamp for the FIRE dynamics (0 = off, the CPU path's
+ * behaviour; the reference GPU update kernel uses 10 eV/A,
+ * relax_multi.cl:980-983).  Convergence is tested on the unclamped force. */
+void set_force_limit(double Flimit);
+/* nSys, ntotA, ntotM, ntotV, maxN, maxM, grid nx, ny, nz */
+void get_batch_sizes(int* out9);
+/* grid pos0[3], step[3], alphaMorse; returns number of coefficients */
+int get_grid_info(double* out7);
+double* get_grid_coefs(void);
+/* Host-only helpers (no device work): generator, topology, synthetic grid. */
+int gen_molecule_arrays(int natoms, long long seed, double sigma, long long pseed, int* nnode_out, double* apos,
+                        double* REQs, int* neighs, double* apars, double* bLs, double* bKs, double* Ksp, double* Kpp,
+                        int* elem);
+int make_back_neighs(int natoms, int nnode, const int* neighs, int* bkneighs);
+int make_exclusions(int natoms, int nnode, const int* neighs, int subtract13, unsigned* mask_out, int* W_out,
+                    int* ncorr_out);
+int make_nacl_gridff(int nx, int ny, int nz, double dz, int nthreads, double* coefs_out, double* geom_out7);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

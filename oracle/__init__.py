@@ -86,9 +86,11 @@ class Backend:
         return f, fng, fngpi, E
 
     def relax(self, m, fire, Fconv, nmax, conv_mode=0, sw=(1, 1, 1, 1, 1), grid=None, ntraj=0):
+        """fire = (dt_max, dt_min, damp_max, finc, fdec, falpha, minLastNeg[, Flimit]);
+        Flimit (per-vec force clamp) is honoured by the oracle only."""
         N, M = m["natoms"], m["nnode"]
         apos = np.array(m["apos"], dtype=np.float64, copy=True)
-        fire = np.ascontiguousarray(np.asarray(fire, dtype=np.float64))
+        fire = np.zeros(8) + np.r_[np.asarray(fire, dtype=np.float64), np.zeros(8 - len(fire))]
         E = ctypes.c_double(0)
         traj = np.zeros((max(ntraj, 1), N + M, 3))
         a, sw_keep = self._sys_args(m, sw)
@@ -164,7 +166,7 @@ def relax_batch(systems, fire, Fconv, nmax, conv_mode=0, sw=(1, 1, 1, 1, 1), gri
     REQs, neighs = cat("REQs"), cat("neighs", np.int32)
     apars, bLs, bKs, Ksp, Kpp = cat("apars"), cat("bLs"), cat("bKs"), cat("Ksp"), cat("Kpp")
     sw = np.ascontiguousarray(np.asarray(sw, np.int32))
-    fire = np.ascontiguousarray(np.asarray(fire, np.float64))
+    fire = np.zeros(8) + np.r_[np.asarray(fire, dtype=np.float64), np.zeros(8 - len(fire))]
     steps = np.zeros(ns, np.int32)
     E = np.zeros(ns)
     Fm = np.zeros(ns)

@@ -434,8 +434,10 @@ void fire_update(Fire& F, double vf, double vv, double ff) {
     }
 }
 
+// fire = {dt_max, dt_min, damp_max, finc, fdec, falpha, minLastNeg, Flimit}
 int relax_sys(Sys& S, const double* fire, double Fconv, int nmax, int conv_mode, double* E_out, double* traj,
               int ntraj, double* fmax_out) {
+    const double Flimit = fire[7];
     const int N = S.N, V_ = S.N + S.M;
     Fire F{};
     F.dt = F.dt_max = fire[0];
@@ -454,12 +456,23 @@ int relax_sys(Sys& S, const double* fire, double Fconv, int nmax, int conv_mode,
         E = eval_sys(S);
         double vf = 0, vv = 0, ff = 0;
         fm2 = 0;
+        for (int i = 0; i < V_; i++) fm2 = std::max(fm2, dot(ld(&S.f[i * 3]), ld(&S.f[i * 3])));
+        if (Flimit > 0) {  // per-vec clamp of the GPU update kernel, relax_multi.cl:980-983
+            for (int i = 0; i < V_; i++) {
+                V f = ld(&S.f[i * 3]);
+                double f2 = dot(f, f);
+                if (f2 > Flimit * Flimit) st(&S.f[i * 3], f * (Flimit / std::sqrt(f2)));
+            }
+        }
         for (int i = 0; i < V_ * 3; i++) {
             vf += vel[i] * S.f[i];
             vv += vel[i] * vel[i];
             ff += S.f[i] * S.f[i];
         }
-        for (int i = 0; i < V_; i++) fm2 = std::max(fm2, dot(ld(&S.f[i * 3]), ld(&S.f[i * 3])));
+        if (!std::isfinite(ff)) {  // blown up: report a non-finite Fmax
+            fm2 = NAN;
+            break;
+        }
         if (conv_mode == 0 && fm2 < F2conv) break;
         fire_update(F, vf, vv, ff);
         // move_atom_FIRE receives dt as float (MMFFsp3_loc.h:1264)
