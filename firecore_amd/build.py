@@ -17,7 +17,7 @@ ARCH = "gfx950"
 # packed FP32 instructions are disabled for all device code
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wno-unused-command-line-argument",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-HIP_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-munsafe-fp-atomics"]
+HIP_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-munsafe-fp-atomics", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
 HIP_SOURCES = ["relax_persistent.hip", "split_kernels.hip", "sample_kernels.hip"]

@@ -68,7 +68,9 @@ __device__ __forceinline__ float angle_cos_half(V3 h1, V3 h2, float ir1, float i
     // cs = cs0 udiv (c,s)  (Vec2.h:130)
     float csx = cs0x * c + cs0y * s;
     float csy = cs0y * c - cs0x * s;
-    float E = k * (1.f - csx);
+    // k*(1 - csx) written as k*csy^2/(1 + csx) (|cs| = 1): no cancellation
+    // near the equilibrium angle, where FP32 1 - csx loses the energy
+    float E = k * csy * csy / (1.f + csx);
     float fr = -k * csy;
     c2 *= -2.f;
     fr /= 4.f * c * s;
@@ -173,17 +175,31 @@ __device__ __forceinline__ float eval_node(int ia, int natoms, int nnode, PosFn 
 struct Req {
     float R, E, kQ, H;  // R0, sqrt(E0), COULOMB_K*Q, H
 };
+// 1/r from one v_rsq_f32 (1 ulp); the H-bond term is compiled in only for
+// batches that carry H-bond charges (HB=true).
+// PRECISE adds one Newton step to 1/r (energy passes: the Coulomb sums cancel
+// to a small fraction of their terms, so a biased 1-ulp 1/r shows up there).
+__device__ __forceinline__ float rsq_nr(float x) {
+    const float y = __builtin_amdgcn_rsqf(x);
+    return y * (1.5f - 0.5f * x * y * y);
+}
+template <bool HB = true, bool PRECISE = false>
 __device__ __forceinline__ void lj_pair(V3 d, float Rij, float Eij, float kQQ, float Hij, float& fr, float& e) {
     const float r2 = dot(d, d);
-    const float ir2 = 1.f / r2;
-    float Eq = kQQ * sqrtf(ir2);
-    float F = Eq * ir2;
-    const float u2 = Rij * Rij * ir2;
+    const float ir = PRECISE ? rsq_nr(r2) : __builtin_amdgcn_rsqf(r2);
+    const float ir2 = ir * ir;
+    const float Eq = kQQ * ir;
+    const float u2 = (Rij * Rij) * ir2;
     const float u6 = u2 * u2 * u2;
     const float vdW = u6 * Eij;
-    const float H = u6 * u6 * ((Hij < 0.f) ? Hij * Eij : 0.f);
-    e = Eq + (u6 - 2.f) * vdW + H;
-    fr = F + ((u6 - 1.f) * vdW + H) * ir2 * 12.f;  // force on i = d * (-fr)
+    if (HB) {
+        const float H = u6 * u6 * ((Hij < 0.f) ? Hij * Eij : 0.f);
+        e = Eq + (u6 - 2.f) * vdW + H;
+        fr = (Eq + ((u6 - 1.f) * vdW + H) * 12.f) * ir2;  // force on i = d * (-fr)
+    } else {
+        e = Eq + (u6 - 2.f) * vdW;
+        fr = (Eq + (u6 - 1.f) * vdW * 12.f) * ir2;
+    }
 }
 
 // -------------------------------------------------------------------------

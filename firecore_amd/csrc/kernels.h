@@ -46,6 +46,7 @@ struct BatchDev {
     const PairCorrDev* corr;
     float4* fneigh;        // [8 * sum M] (split path only)
     float* ebond;          // [sum M] (split path only)
+    double* enb;           // [sum N] per-atom non-bonded + grid energy (split path only)
     // per-system results
     int* nsteps;
     int* status;

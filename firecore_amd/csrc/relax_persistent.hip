@@ -57,9 +57,20 @@ struct LdsLayout {
     __host__ __device__ int bytes() const { return off_mask() + nmax * wmax * 4; }
 };
 
-template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, RunParams P, const int* order,
-                                                                 int nqueue, int* queue, LdsLayout L) {
+// Work queues: class c owns order[qoff[c] .. qoff[c]+qcnt[c]) and counter
+// queue[c].  A workgroup of class `cls` drains its own queue, then steals from
+// the queues of the smaller classes (its LDS layout and width fit them), so no
+// launch idles while another still has systems.
+struct Queues {
+    const int* order;
+    int qoff[4];
+    int qcnt[4];
+    int* counter;
+};
+
+template <int BLOCK, bool HB>
+__global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, RunParams P, Queues Q, int cls,
+                                                                 LdsLayout L) {
     extern __shared__ float4 lds_dyn[];
     char* lds = reinterpret_cast<char*>(lds_dyn);
     float4* s_pos = reinterpret_cast<float4*>(lds + L.off_pos());
@@ -77,13 +88,25 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
     const int lane = tid & 63;
     const int wid = tid >> 6;
 
+    int cc = cls;
     for (;;) {
-        if (tid == 0) s_q = atomicAdd(queue, 1);
+        if (tid == 0) {
+            int q = -1;
+            while (cc >= 0) {
+                const int k = atomicAdd(Q.counter + cc, 1);
+                if (k < Q.qcnt[cc]) {
+                    q = Q.order[Q.qoff[cc] + k];
+                    break;
+                }
+                cc--;
+            }
+            s_q = q;
+        }
         __syncthreads();
         const int q = s_q;
         __syncthreads();
-        if (q >= nqueue) break;
-        const int s = order[q];
+        if (q < 0) break;
+        const int s = q;
         const SysInfo I = B.info[s];
         const int N = I.natoms, M = I.nnode, V = N + M;
         const int W = (N + 31) >> 5;
@@ -136,35 +159,30 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
         float fmax2_last = 0.f;
         V3 fv[2];
         for (;;) {
-            // ================= A: forces
-            float epart = 0.f;
+            // ================= A: forces (no energies inside the step loop)
             if (hasAtom) {
                 const int i = tid;
-                const float4 pi4 = s_pos[i];
-                const V3 pi = xyz(pi4);
+                const V3 pi = xyz(s_pos[i]);
                 const float4 ri = s_req[i];
                 const float kQi = COULOMB_K * ri.z;
                 V3 f = zero3();
-                float e = 0.f;
                 if (P.bNonBonded) {
                     for (int w = 0; w < W; w++) {
                         const unsigned m = s_mask[w * N + i];
                         const int j0 = w << 5;
                         const int jn = min(32, N - j0);
+#pragma unroll 4
                         for (int jj = 0; jj < jn; jj++) {
                             const int j = j0 + jj;
                             const float4 pj = s_pos[j];
                             const float4 rj = s_req[j];
                             const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
                             float fr, ep;
-                            lj_pair(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
-                            const bool ex = (m >> jj) & 1u;
-                            fr = ex ? 0.f : fr;
-                            ep = ex ? 0.f : ep;
+                            lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                            fr = ((m >> jj) & 1u) ? 0.f : fr;
                             f.x -= d.x * fr;
                             f.y -= d.y * fr;
                             f.z -= d.z * fr;
-                            e += ep;
                         }
                     }
                     // weighted corrections (ring topologies; empty for trees)
@@ -175,20 +193,17 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
                         const float4 rj = s_req[pc.j];
                         const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
                         float fr, ep;
-                        lj_pair(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                        lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
                         f -= d * (fr * pc.w);
-                        e += ep * pc.w;
                     }
-                    e *= 0.5f;
                 }
-                if (P.bGridFF) e += grid_sample(P.grid, pi, mk(plq.x, plq.y, plq.z), f);
+                if (P.bGridFF) grid_sample(P.grid, pi, mk(plq.x, plq.y, plq.z), f);
                 s_fnb[i] = make_float4(f.x, f.y, f.z, 0.f);
-                epart += e;
             }
             if (hasNode) {
                 V3 fbs[4], fps[4], fa, fpi;
                 auto pos = [&](int v) { return xyz(s_pos[v]); };
-                epart += eval_node(tid, N, M, pos, ng, apar, bL, bK, ksp, kpp, P.sw, fbs, fps, fa, fpi);
+                eval_node(tid, N, M, pos, ng, apar, bL, bK, ksp, kpp, P.sw, fbs, fps, fa, fpi);
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     s_fng[tid * 4 + k] = make_float4(fbs[k].x, fbs[k].y, fbs[k].z, 0.f);
@@ -261,8 +276,50 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
             else if (crit < P.Fconv2) status = ST_CONVERGED;
             else if (step >= P.nstep_max) status = ST_MAXSTEP;
             if (status != ST_RUNNING) {
-                // FP64 reduction of the energy at this geometry
-                double ed = wave_sum_d((double)epart);
+                // energy of this geometry: precise 1/r, FP64 accumulation and reduction
+                double epart = 0.0;
+                if (hasAtom && P.bNonBonded) {
+                    const int i = tid;
+                    const V3 pi = xyz(s_pos[i]);
+                    const float4 ri = s_req[i];
+                    const float kQi = COULOMB_K * ri.z;
+                    double e = 0.0;
+                    for (int w = 0; w < W; w++) {
+                        const unsigned m = s_mask[w * N + i];
+                        const int j0 = w << 5;
+                        const int jn = min(32, N - j0);
+                        for (int jj = 0; jj < jn; jj++) {
+                            if ((m >> jj) & 1u) continue;
+                            const float4 pj = s_pos[j0 + jj];
+                            const float4 rj = s_req[j0 + jj];
+                            float fr, ep;
+                            lj_pair<true, true>(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), ri.x + rj.x, ri.y * rj.y,
+                                                kQi * rj.z, ri.w * rj.w, fr, ep);
+                            e += (double)ep;
+                        }
+                    }
+                    for (int c = 0; c < I.ncorr; c++) {
+                        const PairCorrDev pc = B.corr[I.corr0 + c];
+                        if (pc.i != i) continue;
+                        const float4 pj = s_pos[pc.j];
+                        const float4 rj = s_req[pc.j];
+                        float fr, ep;
+                        lj_pair<true, true>(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), ri.x + rj.x, ri.y * rj.y,
+                                            kQi * rj.z, ri.w * rj.w, fr, ep);
+                        e += (double)ep * pc.w;
+                    }
+                    epart += 0.5 * e;
+                }
+                if (hasAtom && P.bGridFF) {
+                    V3 fd = zero3();
+                    epart += (double)grid_sample(P.grid, xyz(s_pos[tid]), mk(plq.x, plq.y, plq.z), fd);
+                }
+                if (hasNode) {
+                    V3 fbs[4], fps[4], fa, fpi;
+                    auto pos = [&](int v) { return xyz(s_pos[v]); };
+                    epart += (double)eval_node(tid, N, M, pos, ng, apar, bL, bK, ksp, kpp, P.sw, fbs, fps, fa, fpi);
+                }
+                double ed = wave_sum_d(epart);
                 if (lane == 0) s_redd[wid] = ed;
                 __syncthreads();
                 if (tid == 0) {
@@ -314,34 +371,48 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
     }
 }
 
-template __global__ void relax_persistent_kernel<64>(BatchDev, RunParams, const int*, int, int*, LdsLayout);
-template __global__ void relax_persistent_kernel<128>(BatchDev, RunParams, const int*, int, int*, LdsLayout);
-template __global__ void relax_persistent_kernel<256>(BatchDev, RunParams, const int*, int, int*, LdsLayout);
+#define FC_INST(BL, HB) \
+    template __global__ void relax_persistent_kernel<BL, HB>(BatchDev, RunParams, Queues, int, LdsLayout);
+FC_INST(64, false)
+FC_INST(64, true)
+FC_INST(128, false)
+FC_INST(128, true)
+FC_INST(192, false)
+FC_INST(192, true)
+FC_INST(256, false)
+FC_INST(256, true)
+#undef FC_INST
+
+template <bool HB>
+static const void* kernel_ptr(int block) {
+    switch (block) {
+        case 64: return (const void*)&relax_persistent_kernel<64, HB>;
+        case 128: return (const void*)&relax_persistent_kernel<128, HB>;
+        case 192: return (const void*)&relax_persistent_kernel<192, HB>;
+        case 256: return (const void*)&relax_persistent_kernel<256, HB>;
+        default: return nullptr;
+    }
+}
 
 }  // namespace fcd
 
-// Host launcher (C linkage inside the library).
-extern "C" hipError_t fc_launch_relax_persistent(int block, int ngroups, size_t lds_bytes, hipStream_t stream,
+// Host launcher (C linkage inside the library).  qinfo = {off[4], cnt[4]}.
+extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups, size_t lds_bytes, hipStream_t stream,
                                                  const fcd::BatchDev* B, const fcd::RunParams* P, const int* order,
-                                                 int nqueue, int* queue, int nmax, int mmax, int wmax) {
+                                                 const int* qinfo, int* counters, int cls, int nmax, int mmax,
+                                                 int wmax) {
     fcd::LdsLayout L{nmax, mmax, wmax};
-    switch (block) {
-        case 64:
-            hipLaunchKernelGGL(fcd::relax_persistent_kernel<64>, dim3(ngroups), dim3(64), lds_bytes, stream, *B, *P,
-                               order, nqueue, queue, L);
-            break;
-        case 128:
-            hipLaunchKernelGGL(fcd::relax_persistent_kernel<128>, dim3(ngroups), dim3(128), lds_bytes, stream, *B,
-                               *P, order, nqueue, queue, L);
-            break;
-        case 256:
-            hipLaunchKernelGGL(fcd::relax_persistent_kernel<256>, dim3(ngroups), dim3(256), lds_bytes, stream, *B,
-                               *P, order, nqueue, queue, L);
-            break;
-        default:
-            return hipErrorInvalidValue;
+    fcd::Queues Q;
+    Q.order = order;
+    for (int c = 0; c < 4; c++) {
+        Q.qoff[c] = qinfo[c];
+        Q.qcnt[c] = qinfo[4 + c];
     }
-    return hipGetLastError();
+    Q.counter = counters;
+    const void* k = hb ? fcd::kernel_ptr<true>(block) : fcd::kernel_ptr<false>(block);
+    if (!k) return hipErrorInvalidValue;
+    void* args[] = {(void*)B, (void*)P, (void*)&Q, (void*)&cls, (void*)&L};
+    return hipLaunchKernel(k, dim3(ngroups), dim3(block), args, lds_bytes, stream);
 }
 
 extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax) {
@@ -349,15 +420,8 @@ extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax) {
     return L.bytes();
 }
 
-extern "C" hipError_t fc_persistent_occupancy(int block, size_t lds, int* nblocks) {
-    switch (block) {
-        case 64:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(nblocks, fcd::relax_persistent_kernel<64>, 64, lds);
-        case 128:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(nblocks, fcd::relax_persistent_kernel<128>, 128, lds);
-        case 256:
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(nblocks, fcd::relax_persistent_kernel<256>, 256, lds);
-        default:
-            return hipErrorInvalidValue;
-    }
+extern "C" hipError_t fc_persistent_occupancy(int block, int hb, size_t lds, int* nblocks) {
+    const void* k = hb ? fcd::kernel_ptr<true>(block) : fcd::kernel_ptr<false>(block);
+    if (!k) return hipErrorInvalidValue;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(nblocks, k, block, lds);
 }

@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(64) k_nonbond_grid(BatchDev B, RunParams P) {
     }
     const float kQi = COULOMB_K * ri.z;
     V3 f = zero3();
-    float e = 0.f;
+    double e = 0.0;  // FP64 accumulation of the pair energies
     if (P.bNonBonded) {
         for (int j0 = 0; j0 < N; j0 += 64) {
             const int jl = j0 + threadIdx.x;
@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(64) k_nonbond_grid(BatchDev B, RunParams P) {
                     const float4 rj = s_r[jj];
                     const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
                     float fr, ep;
-                    lj_pair(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                    lj_pair<true, true>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
                     const unsigned m = jj < 32 ? m0 : m1;
                     const bool ex = (m >> (jj & 31)) & 1u;
                     fr = ex ? 0.f : fr;
@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(64) k_nonbond_grid(BatchDev B, RunParams P) {
                     f.x -= d.x * fr;
                     f.y -= d.y * fr;
                     f.z -= d.z * fr;
-                    e += ep;
+                    e += (double)ep;
                 }
             }
         }
@@ -70,19 +70,20 @@ __global__ void __launch_bounds__(64) k_nonbond_grid(BatchDev B, RunParams P) {
                 const float4 rj = B.REQs[I.a0 + pc.j];
                 const V3 d = pj - pi;
                 float fr, ep;
-                lj_pair(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                lj_pair<true, true>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
                 f -= d * (fr * pc.w);
-                e += ep * pc.w;
+                e += (double)ep * pc.w;
             }
         }
-        e *= 0.5f;
+        e *= 0.5;
     }
     if (!act) return;
     if (P.bGridFF) {
         const float4 plq = B.PLQs[I.a0 + i];
-        e += grid_sample(P.grid, pi, mk(plq.x, plq.y, plq.z), f);
+        e += (double)grid_sample(P.grid, pi, mk(plq.x, plq.y, plq.z), f);
     }
-    B.aforce[I.v0 + i] = make_float4(f.x, f.y, f.z, e);
+    B.aforce[I.v0 + i] = make_float4(f.x, f.y, f.z, 0.f);
+    B.enb[I.a0 + i] = e;
 }
 
 // ---------------------------------------------------------------------------
@@ -168,7 +169,7 @@ __global__ void __launch_bounds__(256) k_update(BatchDev B, RunParams P, int do_
     for (int v = threadIdx.x; v < V; v += blockDim.x) {
         const float4 a = B.aforce[I.v0 + v];
         V3 f = xyz(a);
-        if (v < N) e += (double)a.w;
+        if (v < N) e += B.enb[I.a0 + v];
         const int4 b = B.bkNeighs[I.v0 + v];
         if (b.x >= 0) { f += xyz(fng[b.x]);
         if (b.y >= 0) { f += xyz(fng[b.y]);

@@ -12,14 +12,15 @@
 #include <cstdio>
 #include <cstring>
 
-extern "C" hipError_t fc_launch_relax_persistent(int block, int ngroups, size_t lds_bytes, hipStream_t stream,
+extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups, size_t lds_bytes, hipStream_t stream,
                                                  const fcd::BatchDev* B, const fcd::RunParams* P, const int* order,
-                                                 int nqueue, int* queue, int nmax, int mmax, int wmax);
+                                                 const int* qinfo, int* counters, int cls, int nmax, int mmax,
+                                                 int wmax);
 extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax);
 extern "C" hipError_t fc_launch_split_step(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P,
                                            const int* node_sys, int ntot_nodes, int maxN, int do_move, int step);
 extern "C" hipError_t fc_launch_reset(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P);
-extern "C" hipError_t fc_persistent_occupancy(int block, size_t lds, int* nblocks);
+extern "C" hipError_t fc_persistent_occupancy(int block, int hb, size_t lds, int* nblocks);
 
 namespace fcw {
 
@@ -273,6 +274,7 @@ bool World::upload_all(bool bParams, bool bForces, bool bVel) {
         d_node_sys.upload(node_sys, stream);
         d_fneigh.alloc(std::max<size_t>((size_t)ntotM * 8, 1));
         d_ebond.alloc(std::max<size_t>((size_t)ntotM, 1));
+        d_enb.alloc(std::max<size_t>((size_t)ntotA, 1));
         d_fire.alloc(std::max(nSys, 1));
         d_nsteps.upload(nsteps, stream);
         d_status.upload(status, stream);
@@ -353,6 +355,7 @@ fcd::BatchDev World::batch_dev() {
     B.corr = d_corr.p;
     B.fneigh = d_fneigh.p;
     B.ebond = d_ebond.p;
+    B.enb = d_enb.p;
     B.nsteps = d_nsteps.p;
     B.status = d_status.p;
     B.Esys = d_Esys.p;
@@ -385,69 +388,102 @@ fcd::RunParams World::run_params(double Fconv, int nstep_max) {
     return P;
 }
 
-// Group systems by workgroup shape and order them by decreasing cost (LPT),
-// then split the GPU's resident workgroup slots between the classes in
-// proportion to their work so that all classes finish together.
+// Group systems by workgroup width (64/128/192/256 threads: the smallest
+// that holds one atom and one node per thread and two vecs per thread) and
+// order each group by decreasing cost (largest first).  The device's resident
+// workgroup slots are split between the classes in proportion to their
+// estimated time; workgroups that run out of work steal from smaller classes.
+// The estimate uses the previous relaxation's step counts of the same batch
+// when available (the autotuning loop), else a size-based guess.
 void World::plan_classes() {
     classes.clear();
-    const int blocks[3] = {64, 128, 256};
-    std::vector<SizeClass> cl(3);
-    for (int c = 0; c < 3; c++) cl[c].block = blocks[c];
-    std::vector<int> too_big;
+    const int blocks[4] = {64, 128, 192, 256};
+    std::vector<SizeClass> cl(4);
+    for (int c = 0; c < 4; c++) cl[c].block = blocks[c];
+    hb_batch = false;
+    for (const float4& q : REQs)
+        if (q.w < 0.f) hb_batch = true;
+    int too_big = 0;
+    sys_class.assign(nSys, -1);
     for (int s = 0; s < nSys; s++) {
         const int N = info[s].natoms, M = info[s].nnode;
         int c = -1;
-        for (int k = 0; k < 3; k++)
+        for (int k = 0; k < 4; k++)
             if (N <= blocks[k] && M <= blocks[k] && N + M <= 2 * blocks[k]) {
                 c = k;
                 break;
             }
+        sys_class[s] = c;
         if (c < 0) {
-            too_big.push_back(s);
+            too_big++;
             continue;
         }
-        SizeClass& C = cl[c];
-        C.order.push_back(s);
-        C.nmax = std::max(C.nmax, N);
-        C.mmax = std::max(C.mmax, M);
-        C.work += (double)N * N + 40.0 * N;
+        cl[c].order.push_back(s);
+        cl[c].nmax = std::max(cl[c].nmax, N);
+        cl[c].mmax = std::max(cl[c].mmax, M);
     }
-    if (!too_big.empty())
-        std::fprintf(stderr, "[libMMFFmulti_lib] %zu systems exceed 256 atoms and are relaxed by the split path only\n",
-                     too_big.size());
+    if (too_big)
+        std::fprintf(stderr, "[libMMFFmulti_lib] %d systems exceed 256 atoms and are relaxed by the split path only\n",
+                     too_big);
     int ncu = 256;
     {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, 0) == hipSuccess) ncu = prop.multiProcessorCount;
     }
-    double wtot = 0;
-    for (auto& C : cl) wtot += C.work;
-    for (auto& C : cl) {
-        if (C.order.empty()) continue;
+    // LDS of a class covers every system it may steal
+    int nrun = 0, mrun = 0;
+    std::vector<int> qoff(4, 0), qcnt(4, 0);
+    std::vector<int> allorder;
+    for (int c = 0; c < 4; c++) {
+        SizeClass& C = cl[c];
+        nrun = std::max(nrun, C.nmax);
+        mrun = std::max(mrun, C.mmax);
+        C.nmax = nrun;
+        C.mmax = mrun;
         C.wmax = (C.nmax + 31) / 32;
-        C.lds = (size_t)fc_lds_bytes(C.nmax, C.mmax, C.wmax);
+        C.lds = (size_t)fc_lds_bytes(std::max(C.nmax, 1), std::max(C.mmax, 1), std::max(C.wmax, 1));
+        auto cost = [&](int s) {
+            const double N = info[s].natoms;
+            const double st = (nsteps.size() == (size_t)nSys && nsteps[s] > 0) ? nsteps[s] + 1.0 : 1000.0 + 20.0 * N;
+            return st * N * N;
+        };
         std::sort(C.order.begin(), C.order.end(), [&](int a, int b) {
-            double ca = (double)info[a].natoms * info[a].natoms, cb = (double)info[b].natoms * info[b].natoms;
+            double ca = cost(a), cb = cost(b);
             return ca != cb ? ca > cb : a < b;
         });
+        // time estimate: work over the busy fraction of the workgroup's lanes
+        C.work = 0;
+        for (int s : C.order) C.work += cost(s) * C.block / std::max(1, info[s].natoms);
+        qoff[c] = (int)allorder.size();
+        qcnt[c] = (int)C.order.size();
+        allorder.insert(allorder.end(), C.order.begin(), C.order.end());
+    }
+    double wtot = 0;
+    for (auto& C : cl) wtot += C.work;
+    for (int c = 0; c < 4; c++) {
+        SizeClass& C = cl[c];
+        C.cls = c;
+        if (C.order.empty()) continue;
         int nb = 1;
-        fc_persistent_occupancy(C.block, C.lds, &nb);
+        fc_persistent_occupancy(C.block, hb_batch ? 1 : 0, C.lds, &nb);
         nb = std::max(nb, 1);
-        const int slots = nb * ncu;
-        int g = (int)std::lround(slots * C.work / std::max(wtot, 1e-30));
-        g = std::max(1, std::min(g, (int)C.order.size()));
-        C.ngroups = g;
+        C.occ = nb;
+        int g = (int)std::lround((double)ncu * nb * C.work / std::max(wtot, 1e-30));
+        C.ngroups = std::max(1, std::min(g, (int)C.order.size()));
         classes.push_back(C);
     }
-    // device copy of the concatenated orders
-    std::vector<int> allorder;
-    for (auto& C : classes) allorder.insert(allorder.end(), C.order.begin(), C.order.end());
+    qinfo.assign(8, 0);
+    for (int c = 0; c < 4; c++) {
+        qinfo[c] = qoff[c];
+        qinfo[4 + c] = qcnt[c];
+    }
     if (allorder.empty()) allorder.push_back(0);
     d_order.upload(allorder, stream);
 }
 
 int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
     if (!dev_ready || nSys == 0) return -1;
+    if (autotune) plan_classes();
     if (restore_start)
         hip_ok(hipMemcpyAsync(d_atoms.p, d_atoms0.p, sizeof(float4) * ntotV, hipMemcpyDeviceToDevice, stream),
                "restore");
@@ -460,23 +496,21 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
     hipEventCreate(&e1);
     hipEventRecord(e0, stream);
     std::vector<hipEvent_t> starts, done;
-    int off = 0;
-    for (size_t c = 0; c < classes.size(); c++) {
-        SizeClass& C = classes[c];
-        hipStream_t st = class_streams[c % class_streams.size()];
+    for (size_t k = 0; k < classes.size(); k++) {
+        SizeClass& C = classes[k];
+        hipStream_t st = class_streams[k % class_streams.size()];
         hipStreamWaitEvent(st, e0, 0);
         hipEvent_t es, ev;
         hipEventCreate(&es);
         hipEventCreate(&ev);
         hipEventRecord(es, st);
-        hipError_t err = fc_launch_relax_persistent(C.block, C.ngroups, C.lds, st, &B, &P, d_order.p + off,
-                                                    (int)C.order.size(), d_queue.p + c, C.nmax, C.mmax, C.wmax);
+        hipError_t err = fc_launch_relax_persistent(C.block, hb_batch ? 1 : 0, C.ngroups, C.lds, st, &B, &P, d_order.p,
+                                                    qinfo.data(), d_queue.p, C.cls, C.nmax, C.mmax, C.wmax);
         if (!hip_ok(err, "relax_persistent launch")) return -1;
         hipEventRecord(ev, st);
         hipStreamWaitEvent(stream, ev, 0);
         starts.push_back(es);
         done.push_back(ev);
-        off += (int)C.order.size();
     }
     hipEventRecord(e1, stream);
     if (!hip_ok(hipStreamSynchronize(stream), "relax_persistent")) return -1;

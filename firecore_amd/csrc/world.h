@@ -26,6 +26,8 @@ struct DevArr {
 // workgroup shape share one launch.
 struct SizeClass {
     int block = 256;
+    int cls = 0;   // index among the four widths (queue index)
+    int occ = 1;   // resident workgroups per CU
     int nmax = 0, mmax = 0, wmax = 0;
     std::vector<int> order;  // system indices, decreasing cost
     double work = 0;         // sum of N^2
@@ -82,9 +84,12 @@ struct World {
     DevArr<fcd::PairCorrDev> d_corr;
     DevArr<fcd::SysInfo> d_info;
     DevArr<int> d_node_sys, d_nsteps, d_status, d_order, d_queue;
-    DevArr<double> d_Esys;
+    DevArr<double> d_Esys, d_enb;
     DevArr<float> d_Fmax, d_ebond;
     std::vector<SizeClass> classes;
+    std::vector<int> qinfo, sys_class;
+    bool hb_batch = false;
+    bool autotune = true;
     double last_ms = 0;
 
     // ---- buffer registry (libUtils.h:16-66)

@@ -1,5 +1,11 @@
 """GPU parity of the HIP path against the FP64 oracle (through the C-ABI).
 
+The oracle receives the exact double-precision inputs where the energy is
+compared at relaxed geometries: the device stores parameters in FP32, and the
+rounded (cos, sin) of the equilibrium angle is not unit-norm, which offsets
+the reference's k*(1-cos) angle energy by k*(1-|cs0|) per angle (~2e-4 eV for
+250-atom systems); the device uses the equivalent form k*csy^2/(1+csx).
+
 Tolerances (FP32 device vs FP64 oracle):
   forces    max|dF| <= 1e-4 eV/A + 5e-6 * max|F|   (strained start geometries
             carry |F| up to ~10^3 eV/A; at relaxed geometries the bound is
@@ -131,8 +137,10 @@ def test_relax_config1_replicas(nat, orc):
     B = nat.Batch()
     assert (B.status == nat.ST_CONVERGED).all(), np.bincount(B.status, minlength=4)
     assert (B.Fmax < 1e-3).all()
+    tmpl = nat.gen_molecule(24, 0)  # exact (double) parameters shared by all replicas
     for s in range(0, 1024, 97):
-        m = B.system(s)
+        m = dict(tmpl)
+        m["apos"] = B.positions(s)
         fo, _, _, Eo = orc.eval(m)
         assert fmax(fo) < 1.01e-3, (s, fmax(fo))
         assert abs(B.E[s] - Eo) < 1e-4, (s, B.E[s], Eo)
@@ -151,7 +159,9 @@ def test_relax_gridff_subset(nat, orc):
     conv = np.where(B.status == nat.ST_CONVERGED)[0]
     assert len(conv) > 100, np.bincount(B.status, minlength=4)
     for s in conv[::9]:
-        m = B.system(s)
+        # the exact (double) inputs of system s at the GPU's final positions
+        m = nat.gen_workload_system(1, int(s), 10001, 16, 256, 0.1, g)
+        m["apos"] = B.positions(s)
         fo, _, _, Eo = orc.eval(m, grid=g)
         assert fmax(fo) < 1.01e-3, (s, fmax(fo))
         assert abs(B.E[s] - Eo) < 1e-4, (s, B.E[s], Eo)
