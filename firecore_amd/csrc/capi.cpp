@@ -328,6 +328,7 @@ void setSwitches_multi(int CheckInvariants, int PBC, int NonBonded, int MMFF, in
         W.rebuild_exclusions();
         if (W.dev_ready) {
             W.d_excl.upload(W.excl, W.stream);
+            W.d_excl_rot.upload(W.excl_rot, W.stream);
             W.d_corr.upload(W.corr, W.stream);
             W.d_info.upload(W.info, W.stream);
             W.sync();

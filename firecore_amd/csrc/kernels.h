@@ -43,6 +43,7 @@ struct BatchDev {
     const float4* Ksp;
     const float4* Kpp;
     const unsigned* excl;  // exclusion bit-matrix rows
+    const unsigned* excl_rot;  // rotated rows: bit k of row i = pair (i, (i+k) mod N)
     const PairCorrDev* corr;
     float4* fneigh;        // [8 * sum M] (split path only)
     float* ebond;          // [sum M] (split path only)

@@ -45,13 +45,21 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 }
 
 // LDS layout (bytes), computed identically on the host (world.cpp).
+//   pos   float4[nmax+mmax]     positions of atoms and pi-vectors
+//   req   float4[nmax]          (R0, sqrt(E0), Q, H)
+//   fnb   float4[nmax]          non-bonded + grid force accumulated by atom i's thread
+//   facc  float4[nwave*nmax]    per-wave reaction forces of the half-ring schedule
+//   fself float4[2*mmax]        bonded self forces of nodes and pi-vectors
+//   fng   float4[8*mmax]        recoils (sigma rows, then pi rows)
+//   mask  uint[wmax*nmax]       rotated exclusion rows, [word][atom]
 struct LdsLayout {
-    int nmax, mmax, wmax;
+    int nmax, mmax, wmax, nwave;
     __host__ __device__ int vmax() const { return nmax + mmax; }
     __host__ __device__ int off_pos() const { return 0; }
     __host__ __device__ int off_req() const { return off_pos() + vmax() * 16; }
     __host__ __device__ int off_fnb() const { return off_req() + nmax * 16; }
-    __host__ __device__ int off_fself() const { return off_fnb() + nmax * 16; }
+    __host__ __device__ int off_facc() const { return off_fnb() + nmax * 16; }
+    __host__ __device__ int off_fself() const { return off_facc() + nwave * nmax * 16; }
     __host__ __device__ int off_fng() const { return off_fself() + 2 * mmax * 16; }
     __host__ __device__ int off_mask() const { return off_fng() + 8 * mmax * 16; }
     __host__ __device__ int bytes() const { return off_mask() + nmax * wmax * 4; }
@@ -79,6 +87,7 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
     float4* s_fself = reinterpret_cast<float4*>(lds + L.off_fself());
     float4* s_fng = reinterpret_cast<float4*>(lds + L.off_fng());
     unsigned* s_mask = reinterpret_cast<unsigned*>(lds + L.off_mask());
+    float4* s_facc = reinterpret_cast<float4*>(lds + L.off_facc());
     constexpr int NW = BLOCK / 64;
     __shared__ float s_red[4][NW];
     __shared__ double s_redd[NW];
@@ -115,7 +124,7 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
         for (int v = tid; v < V; v += BLOCK) s_pos[v] = B.atoms[I.v0 + v];
         for (int i = tid; i < N; i += BLOCK) s_req[i] = B.REQs[I.a0 + i];
         // transposed to [word][atom] so that lanes read consecutive banks
-        for (int w = tid; w < N * W; w += BLOCK) s_mask[(w % W) * N + (w / W)] = B.excl[I.mask0 + w];
+        for (int w = tid; w < N * W; w += BLOCK) s_mask[(w % W) * N + (w / W)] = B.excl_rot[I.mask0 + w];
 
         // ---- per-thread persistent state: atom tid, node tid, vecs tid and tid+BLOCK
         const bool hasAtom = tid < N;
@@ -160,6 +169,15 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
         V3 fv[2];
         for (;;) {
             // ================= A: forces (no energies inside the step loop)
+            // Half-ring pair schedule (Newton's third law): thread i takes the
+            // partners j = (i+k) mod N, k = 1..N/2, so every unordered pair is
+            // computed once.  The reaction -f_ij goes to the wave's private
+            // accumulator row: within one instruction the lanes of a wave hold
+            // distinct j, and no other wave writes that row.
+            const int nwAct = (N + 63) >> 6;
+            float4* facc = s_facc + wid * L.nmax;
+            if (wid < nwAct)
+                for (int j = lane; j < N; j += 64) facc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (hasAtom) {
                 const int i = tid;
                 const V3 pi = xyz(s_pos[i]);
@@ -167,22 +185,48 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
                 const float kQi = COULOMB_K * ri.z;
                 V3 f = zero3();
                 if (P.bNonBonded) {
-                    for (int w = 0; w < W; w++) {
+                    const int K = (N - 1) >> 1;
+                    for (int w = 0; (w << 5) <= K; w++) {
                         const unsigned m = s_mask[w * N + i];
-                        const int j0 = w << 5;
-                        const int jn = min(32, N - j0);
+                        const int k0 = max(1, w << 5);
+                        const int k1 = min(K, (w << 5) + 31);
 #pragma unroll 4
-                        for (int jj = 0; jj < jn; jj++) {
-                            const int j = j0 + jj;
+                        for (int k = k0; k <= k1; k++) {
+                            int j = i + k;
+                            j = (j >= N) ? j - N : j;
                             const float4 pj = s_pos[j];
                             const float4 rj = s_req[j];
                             const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
                             float fr, ep;
                             lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
-                            fr = ((m >> jj) & 1u) ? 0.f : fr;
+                            fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
                             f.x -= d.x * fr;
                             f.y -= d.y * fr;
                             f.z -= d.z * fr;
+                            float4 a = facc[j];
+                            a.x += d.x * fr;
+                            a.y += d.y * fr;
+                            a.z += d.z * fr;
+                            facc[j] = a;
+                        }
+                    }
+                    if (!(N & 1)) {  // k = N/2 pairs (i, i+N/2): taken by the lower half only
+                        const int k = N >> 1;
+                        if (i < k) {
+                            const int j = i + k;
+                            const unsigned m = s_mask[(k >> 5) * N + i];
+                            const float4 pj = s_pos[j];
+                            const float4 rj = s_req[j];
+                            const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                            float fr, ep;
+                            lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                            fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
+                            f -= d * fr;
+                            float4 a = facc[j];
+                            a.x += d.x * fr;
+                            a.y += d.y * fr;
+                            a.z += d.z * fr;
+                            facc[j] = a;
                         }
                     }
                     // weighted corrections (ring topologies; empty for trees)
@@ -225,6 +269,7 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
                     const bool isPi = v >= N;
                     if (!isPi) {
                         f = xyz(s_fnb[v]);
+                        for (int w = 0; w < nwAct; w++) f += xyz(s_facc[w * L.nmax + v]);
                         if (v < M) f += xyz(s_fself[v]);
                     } else {
                         f = xyz(s_fself[M + (v - N)]);
@@ -283,21 +328,21 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
                     const V3 pi = xyz(s_pos[i]);
                     const float4 ri = s_req[i];
                     const float kQi = COULOMB_K * ri.z;
-                    double e = 0.0;
-                    for (int w = 0; w < W; w++) {
-                        const unsigned m = s_mask[w * N + i];
-                        const int j0 = w << 5;
-                        const int jn = min(32, N - j0);
-                        for (int jj = 0; jj < jn; jj++) {
-                            if ((m >> jj) & 1u) continue;
-                            const float4 pj = s_pos[j0 + jj];
-                            const float4 rj = s_req[j0 + jj];
-                            float fr, ep;
-                            lj_pair<true, true>(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), ri.x + rj.x, ri.y * rj.y,
-                                                kQi * rj.z, ri.w * rj.w, fr, ep);
-                            e += (double)ep;
-                        }
+                    double e = 0.0;  // each unordered pair once (half ring)
+                    const int Kh = N >> 1;
+                    for (int k = 1; k <= Kh; k++) {
+                        if (!(N & 1) && k == Kh && i >= Kh) break;
+                        if ((s_mask[(k >> 5) * N + i] >> (k & 31)) & 1u) continue;
+                        int j = i + k;
+                        j = (j >= N) ? j - N : j;
+                        const float4 pj = s_pos[j];
+                        const float4 rj = s_req[j];
+                        float fr, ep;
+                        lj_pair<true, true>(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), ri.x + rj.x, ri.y * rj.y,
+                                            kQi * rj.z, ri.w * rj.w, fr, ep);
+                        e += (double)ep;
                     }
+                    double ec = 0.0;  // directed corrections carry half of the pair energy each
                     for (int c = 0; c < I.ncorr; c++) {
                         const PairCorrDev pc = B.corr[I.corr0 + c];
                         if (pc.i != i) continue;
@@ -306,9 +351,9 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
                         float fr, ep;
                         lj_pair<true, true>(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), ri.x + rj.x, ri.y * rj.y,
                                             kQi * rj.z, ri.w * rj.w, fr, ep);
-                        e += (double)ep * pc.w;
+                        ec += (double)ep * pc.w;
                     }
-                    epart += 0.5 * e;
+                    epart += e + 0.5 * ec;
                 }
                 if (hasAtom && P.bGridFF) {
                     V3 fd = zero3();
@@ -401,7 +446,7 @@ extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups,
                                                  const fcd::BatchDev* B, const fcd::RunParams* P, const int* order,
                                                  const int* qinfo, int* counters, int cls, int nmax, int mmax,
                                                  int wmax) {
-    fcd::LdsLayout L{nmax, mmax, wmax};
+    fcd::LdsLayout L{nmax, mmax, wmax, block / 64};
     fcd::Queues Q;
     Q.order = order;
     for (int c = 0; c < 4; c++) {
@@ -415,8 +460,8 @@ extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups,
     return hipLaunchKernel(k, dim3(ngroups), dim3(block), args, lds_bytes, stream);
 }
 
-extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax) {
-    fcd::LdsLayout L{nmax, mmax, wmax};
+extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block) {
+    fcd::LdsLayout L{nmax, mmax, wmax, block / 64};
     return L.bytes();
 }
 

@@ -75,4 +75,22 @@ void make_exclusions(int natoms, int nnode, const int* neighs, bool subtract13,
     }
 }
 
+// Rotated exclusion rows for the half-ring pair schedule: bit k of row i is
+// the exclusion bit of the pair (i, (i+k) mod N).  Returns false when the
+// exclusions are not symmetric (then every pair needs both directions).
+bool rotate_exclusions(int natoms, const std::vector<uint32_t>& mask, int W, std::vector<uint32_t>& rot) {
+    rot.assign((size_t)natoms * W, 0u);
+    bool sym = true;
+    for (int i = 0; i < natoms; i++) {
+        for (int k = 0; k < natoms; k++) {
+            const int j = (i + k) % natoms;
+            const bool b = (mask[(size_t)i * W + (j >> 5)] >> (j & 31)) & 1u;
+            const bool bt = (mask[(size_t)j * W + (i >> 5)] >> (i & 31)) & 1u;
+            if (b != bt) sym = false;
+            if (b || bt) rot[(size_t)i * W + (k >> 5)] |= 1u << (k & 31);
+        }
+    }
+    return sym;
+}
+
 }  // namespace fc

@@ -16,7 +16,7 @@ extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups,
                                                  const fcd::BatchDev* B, const fcd::RunParams* P, const int* order,
                                                  const int* qinfo, int* counters, int cls, int nmax, int mmax,
                                                  int wmax);
-extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax);
+extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block);
 extern "C" hipError_t fc_launch_split_step(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P,
                                            const int* node_sys, int ntot_nodes, int maxN, int do_move, int step);
 extern "C" hipError_t fc_launch_reset(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P);
@@ -82,7 +82,9 @@ bool World::set_systems(std::vector<fc::MolSys>&& ms) {
 
 void World::rebuild_exclusions() {
     excl.clear();
+    excl_rot.clear();
     corr.clear();
+    ring_ok = true;
     const bool sub13 = bNonBonded && sw.doAngles;
     for (int s = 0; s < nSys; s++) {
         const fc::MolSys& m = mols[s];
@@ -94,6 +96,9 @@ void World::rebuild_exclusions() {
         info[s].corr0 = (int)corr.size();
         info[s].ncorr = (int)pc.size();
         excl.insert(excl.end(), mask.begin(), mask.end());
+        std::vector<uint32_t> rot;
+        if (!fc::rotate_exclusions(m.natoms, mask, W, rot)) ring_ok = false;
+        excl_rot.insert(excl_rot.end(), rot.begin(), rot.end());
         for (auto& c : pc) corr.push_back({c.i, c.j, c.w, 0.f});
     }
 }
@@ -269,6 +274,7 @@ bool World::upload_all(bool bParams, bool bForces, bool bVel) {
         d_Ksp.upload(Ksp, stream);
         d_Kpp.upload(Kpp, stream);
         d_excl.upload(excl, stream);
+        d_excl_rot.upload(excl_rot, stream);
         d_corr.upload(corr, stream);
         d_info.upload(info, stream);
         d_node_sys.upload(node_sys, stream);
@@ -352,6 +358,7 @@ fcd::BatchDev World::batch_dev() {
     B.Ksp = d_Ksp.p;
     B.Kpp = d_Kpp.p;
     B.excl = d_excl.p;
+    B.excl_rot = d_excl_rot.p;
     B.corr = d_corr.p;
     B.fneigh = d_fneigh.p;
     B.ebond = d_ebond.p;
@@ -441,7 +448,7 @@ void World::plan_classes() {
         C.nmax = nrun;
         C.mmax = mrun;
         C.wmax = (C.nmax + 31) / 32;
-        C.lds = (size_t)fc_lds_bytes(std::max(C.nmax, 1), std::max(C.mmax, 1), std::max(C.wmax, 1));
+        C.lds = (size_t)fc_lds_bytes(std::max(C.nmax, 1), std::max(C.mmax, 1), std::max(C.wmax, 1), C.block);
         auto cost = [&](int s) {
             const double N = info[s].natoms;
             const double st = (nsteps.size() == (size_t)nSys && nsteps[s] > 0) ? nsteps[s] + 1.0 : 1000.0 + 20.0 * N;
@@ -483,6 +490,10 @@ void World::plan_classes() {
 
 int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
     if (!dev_ready || nSys == 0) return -1;
+    if (!ring_ok) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] asymmetric exclusions: relaxing with the split kernels\n");
+        return relax_split(nstep_max, Fconv, restore_start);
+    }
     if (autotune) plan_classes();
     if (restore_start)
         hip_ok(hipMemcpyAsync(d_atoms.p, d_atoms0.p, sizeof(float4) * ntotV, hipMemcpyDeviceToDevice, stream),

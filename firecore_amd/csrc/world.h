@@ -57,7 +57,8 @@ struct World {
     // ---- host mirrors in the reference's packed float layout
     std::vector<float4> atoms, aforces, avel, REQs, PLQs, MMpars, BLs, BKs, Ksp, Kpp, constr, lvecs, ilvecs, pbcshifts;
     std::vector<int4> neighs, neighCell, bkNeighs;
-    std::vector<unsigned> excl;
+    std::vector<unsigned> excl, excl_rot;
+    bool ring_ok = true;
     std::vector<fcd::PairCorrDev> corr;
     std::vector<int> node_sys, elem;
     std::vector<int> nsteps, status;
@@ -80,7 +81,7 @@ struct World {
     DevArr<float4> d_atoms, d_atoms0, d_aforce, d_avel, d_REQs, d_PLQs, d_apars, d_bLs, d_bKs, d_Ksp, d_Kpp, d_fneigh,
         d_fire, d_grid;
     DevArr<int4> d_neighs, d_bkNeighs;
-    DevArr<unsigned> d_excl;
+    DevArr<unsigned> d_excl, d_excl_rot;
     DevArr<fcd::PairCorrDev> d_corr;
     DevArr<fcd::SysInfo> d_info;
     DevArr<int> d_node_sys, d_nsteps, d_status, d_order, d_queue;
