@@ -793,6 +793,172 @@ void set_conv_mode(int mode) { world().conv_mode = mode; }
 // path's behaviour).  Convergence is always tested on the unclamped force.
 void set_force_limit(double Flimit) { world().Flimit = (float)Flimit; }
 
+// FIRE force preconditioner of the persistent path: mode 0 off (the
+// reference dynamics), 1 rigid-group projections with constant gains
+// (gstar per bonded star, grigid per molecule), 2 gains predicted by the
+// proposal network every `every` steps with gstar/grigid as ceilings.
+// Returns 0, or -1 for an invalid mode or gain.
+int set_precond(int mode, double gstar, double grigid) {
+    if (mode < 0 || mode > 2 || !(gstar >= 0.0) || !(grigid >= 0.0)) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] set_precond: invalid mode %d or gains %g %g\n", mode, gstar, grigid);
+        return -1;
+    }
+    World& W = world();
+    if (mode == 2 && !W.d_mlpW.p) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] set_precond: mode 2 needs proposal weights (set_proposal_weights)\n");
+        return -1;
+    }
+    W.pc_mode = mode;
+    W.pc_gstar = (float)gstar;
+    W.pc_grigid = (float)grigid;
+    if (W.dev_ready && W.nSys > 0) W.plan_classes();
+    return 0;
+}
+
+// Exact forces of every system (download into gpu_aforces) and the
+// preconditioned force of the current set_precond gains (download into the
+// additive view gpu_pcforces).  0 on success.
+int eval_precond() {
+    World& W = world();
+    return W.eval_precond() ? 0 : -1;
+}
+
+// ---- proposal network (no counterpart in the reference) ----------------
+// Parameters in the packing order W1[64][16] b1[64] W2[64][128] b2[64]
+// W3[64][128] b3[64] W4[2][64] b4[2] (FP32); dense weights are used in bf16.
+int get_proposal_nparam() { return 64 * 16 + 64 + 2 * (64 * 128 + 64) + 128 + 2; }
+int set_proposal_weights(const float* theta, int n) {
+    World& W = world();
+    if (!theta || !W.set_mlp(theta, n)) return -1;
+    W.theta_dev = false;  // training restarts from these parameters
+    return 0;
+}
+int get_proposal_weights(float* theta, int n) {
+    World& W = world();
+    if (!W.download_theta()) return -1;
+    if ((int)W.mlp_theta.size() != n || !theta) return -1;
+    std::memcpy(theta, W.mlp_theta.data(), sizeof(float) * n);
+    return 0;
+}
+// steps between network evaluations inside the relaxation (>= 1)
+int set_proposal_every(int every) {
+    if (every < 1) return -1;
+    world().pc_every = every;
+    return 0;
+}
+// Record trajectory snapshots during the next relax() calls: positions of
+// every vec at steps 0, every, 2*every, ... (n snapshots at most).  n = 0
+// stops recording.  0 on success.
+int record_snapshots(int every, int n) { return world().record_snapshots(every, n) ? 0 : -1; }
+// Load snapshot idx[s] of every system s into its positions.
+int load_snapshots(const int* idx) {
+    World& W = world();
+    std::vector<int> v(idx, idx + W.nSys);
+    return W.load_snapshots(v) ? 0 : -1;
+}
+// Forward pass of the proposal network at the current positions (exact
+// forces are evaluated first).  out: [ntotA][2] outputs; feats (optional):
+// [ntotA][16] the inputs as fed to the matrix cores (bf16 values as float).
+int proposal_forward(float* out, float* feats) {
+    World& W = world();
+    if (!W.prop_prepare() || !W.prop_forward()) return -1;
+    std::vector<float> o((size_t)W.ntotA * 2);
+    W.d_out.download(o, W.stream);
+    std::vector<unsigned short> x;
+    if (feats) {
+        x.resize((size_t)W.ntotA * 16);
+        W.d_X.download(x, W.stream);
+    }
+    if (!fcw::hip_ok(hipStreamSynchronize(W.stream), "proposal_forward")) return -1;
+    if (out) std::memcpy(out, o.data(), sizeof(float) * o.size());
+    if (feats)
+        for (size_t i = 0; i < x.size(); i++) {
+            const uint32_t u = ((uint32_t)x[i]) << 16;
+            std::memcpy(feats + i, &u, 4);
+        }
+    return 0;
+}
+// Training loss of one preconditioned step from the current positions after
+// proposal_forward: per-system loss (E(x0 + eta*F~) - E(x0)) / N into loss
+// [nSys], dloss/doutputs into dout [ntotA][2] (either may be NULL).
+int proposal_loss_grad(double eta, double dmax, float* dout, float* loss) {
+    World& W = world();
+    if (!W.prop_loss_grad((float)eta, (float)dmax)) return -1;
+    std::vector<float> d((size_t)W.ntotA * 2), l(W.nSys);
+    W.d_dout.download(d, W.stream);
+    W.d_loss.download(l, W.stream);
+    if (!fcw::hip_ok(hipStreamSynchronize(W.stream), "proposal_loss_grad")) return -1;
+    if (dout) std::memcpy(dout, d.data(), sizeof(float) * d.size());
+    if (loss) std::memcpy(loss, l.data(), sizeof(float) * l.size());
+    return 0;
+}
+// Backward pass after proposal_forward for the given dloss/doutputs
+// ([ntotA][2]); grad [nparam] receives the mean over systems of the
+// per-system parameter gradients.
+int proposal_backward(const float* dout, float* grad) {
+    World& W = world();
+    if (!W.d_dout.p || !dout) return -1;
+    std::vector<float> d(dout, dout + (size_t)W.ntotA * 2);
+    W.d_dout.upload(d, W.stream);
+    if (!W.prop_backward()) return -1;
+    std::vector<float> g(get_proposal_nparam());
+    W.d_grad.download(g, W.stream);
+    if (!fcw::hip_ok(hipStreamSynchronize(W.stream), "proposal_backward")) return -1;
+    if (grad) std::memcpy(grad, g.data(), sizeof(float) * g.size());
+    return 0;
+}
+// Train the proposal network on the recorded snapshots: iters steps of Adam
+// (learning rate lr), each on one snapshot per system drawn uniformly from
+// the system's valid snapshots (counts from the last relax).  loss_hist
+// (optional, [iters]) receives the mean loss of every step.
+int train_proposal(int iters, double lr, double eta, double dmax, long long seed, float* loss_hist) {
+    World& W = world();
+    if (!W.snap_on || W.nsnap < 1 || iters < 0) return -1;
+    std::vector<int> cnt(W.nSys);
+    for (int s = 0; s < W.nSys; s++)
+        cnt[s] = std::max(1, std::min(W.nsnap, W.nsteps[s] / W.snap_every + 1));
+    fc::Rng rng((uint64_t)seed);
+    std::vector<int> idx(W.nSys);
+    std::vector<float> lm(1);
+    for (int it = 0; it < iters; it++) {
+        for (int s = 0; s < W.nSys; s++) idx[s] = (int)(rng.next() % (uint64_t)cnt[s]);
+        if (!W.load_snapshots(idx) || !W.prop_prepare() || !W.prop_forward() ||
+            !W.prop_loss_grad((float)eta, (float)dmax) || !W.prop_backward() || !W.prop_adam((float)lr))
+            return -1;
+        if (loss_hist) {
+            W.d_lossmean.download(lm, W.stream);
+            if (!fcw::hip_ok(hipStreamSynchronize(W.stream), "train")) return -1;
+            loss_hist[it] = lm[0];
+        }
+    }
+    W.restore_positions();
+    return W.download_theta() ? 0 : -1;
+}
+
+// Chunk iterations of the last learned relaxation; 0 after other paths.
+int get_last_chunks() { return world().last_chunks; }
+// 1 = replay the learned-relaxation chunk loop from a captured graph
+// (default), 0 = launch every chunk directly.  Returns the previous setting.
+int set_graphs(int on) {
+    World& W = world();
+    const int old = W.use_graphs ? 1 : 0;
+    W.use_graphs = on != 0;
+    return old;
+}
+
+// Workgroup scheduling of the persistent path: 0 = device slots split between
+// the size classes by estimated time with work stealing, 1 = every class
+// launched at full device width (the dispatcher interleaves them), no
+// stealing.  Returns the previous mode, or -1 for an invalid mode.
+int set_sched(int mode) {
+    World& W = world();
+    if (mode < 0 || mode > 1) return -1;
+    const int old = W.sched_mode;
+    W.sched_mode = mode;
+    if (W.dev_ready && W.nSys > 0) W.plan_classes();
+    return old;
+}
+
 // nSys, ntotA, ntotM, ntotV, maxN, maxM, grid nx, ny, nz
 void get_batch_sizes(int* out) {
     World& W = world();

@@ -87,6 +87,7 @@ void make_exclusions(int natoms, int nnode, const int* neighs, bool subtract13,
                      std::vector<PairCorr>& corr);
 // Rotated rows (bit k of row i = pair (i, (i+k) mod N)) for the half-ring
 // schedule of the persistent kernel; false if the exclusions are asymmetric.
+int connected_components(int natoms, const int* neighs, int maxc, int* comp);
 bool rotate_exclusions(int natoms, const std::vector<uint32_t>& mask, int W, std::vector<uint32_t>& rot);
 
 // ---------------- GridFF (gridff_host.cpp) ----------------

@@ -23,6 +23,9 @@ __device__ __forceinline__ void operator+=(V3& a, V3 b) { a.x += b.x; a.y += b.y
 __device__ __forceinline__ void operator-=(V3& a, V3 b) { a.x -= b.x; a.y -= b.y; a.z -= b.z; }
 __device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ V3 zero3() { return V3{0.f, 0.f, 0.f}; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
 
 // Switches of the bonded terms (setSwitches_multi, MMFFmulti_lib.cpp:316-329).
 struct FFSwitch {

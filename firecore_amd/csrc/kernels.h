@@ -54,6 +54,29 @@ struct BatchDev {
     double* Esys;
     float* Fmax;
     float4* fire;          // per-system FIRE state (dt, damping, lastNeg, -) for the split path
+    // preconditioner / proposal network inputs
+    const int* comp;       // [sum N] connected-component index of each atom (< PC_MAXC)
+    const int* ncomp;      // [nsys] number of components
+    const int* elem;       // [sum N] element class 0..3 (C, N, O, H)
+    // trajectory snapshots (training data): positions of every vec at steps
+    // 0, snap_every, 2*snap_every, ... into snap[k * ntotV + v0 + v], k < nsnap
+    float4* snap;
+    int snap_every, nsnap, ntotV;
+    // learned preconditioner gains (written by the proposal network)
+    const float* gnode;    // [sum M] star gains
+    const float* gcomp;    // [nsys * PC_MAXC] molecule gains
+};
+
+// Parameters of the proposal network (BF16 dense layers, FP32 head).
+struct MlpDev {
+    const unsigned short* W1;  // [64][16]
+    const unsigned short* W2;  // [64][128]
+    const unsigned short* W3;  // [64][128]
+    const float* b1;           // [64]
+    const float* b2;
+    const float* b3;
+    const float* W4;           // [2][64]
+    const float* b4;           // [2]
 };
 
 struct RunParams {
@@ -66,6 +89,15 @@ struct RunParams {
     float Flimit;  // >0: clamp |f| per vec for the dynamics (relax_multi.cl:980-983); 0: off
     int conv_mode;
     int nstep_max;
+    // FIRE force preconditioner: 0 off, 1 constant gains, 2 gains from the proposal network
+    int pc_mode;
+    float pc_gstar, pc_grigid;  // constant gains (mode 1) or gain ceilings (mode 2)
+    int pc_every;               // mode 2: steps between network evaluations
+    MlpDev mlp;
+    // chunked execution: > 0 = every system runs at most `chunk` steps per
+    // launch, resuming from (and suspending to) the state in B (positions,
+    // avel, fire, nsteps, status); 0 = each system runs to completion
+    int chunk;
 };
 
 }  // namespace fcd

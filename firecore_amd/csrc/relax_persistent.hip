@@ -20,6 +20,7 @@
 //   C. per-system FIRE (DynamicOpt.cpp:250-267) and move (move_atom_FIRE,
 //      MMFFsp3_loc.h:1264-1281)
 #include "kernels.h"
+#include "precond.h"
 
 namespace fcd {
 
@@ -27,22 +28,6 @@ template <int BLOCK>
 struct BlockRed {
     static constexpr int NW = BLOCK / 64;
 };
-
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 // LDS layout (bytes), computed identically on the host (world.cpp).
 //   pos   float4[nmax+mmax]     positions of atoms and pi-vectors
@@ -52,8 +37,10 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 //   fself float4[2*mmax]        bonded self forces of nodes and pi-vectors
 //   fng   float4[8*mmax]        recoils (sigma rows, then pi rows)
 //   mask  uint[wmax*nmax]       rotated exclusion rows, [word][atom]
+//   gain  float[mmax+PC_MAXC]   learned gains of the stars and the components
 struct LdsLayout {
-    int nmax, mmax, wmax, nwave;
+    int nmax, mmax, wmax, nwave, learned;
+    __host__ __device__ static int r16(int b) { return (b + 15) & ~15; }
     __host__ __device__ int vmax() const { return nmax + mmax; }
     __host__ __device__ int off_pos() const { return 0; }
     __host__ __device__ int off_req() const { return off_pos() + vmax() * 16; }
@@ -62,7 +49,9 @@ struct LdsLayout {
     __host__ __device__ int off_fself() const { return off_facc() + nwave * nmax * 16; }
     __host__ __device__ int off_fng() const { return off_fself() + 2 * mmax * 16; }
     __host__ __device__ int off_mask() const { return off_fng() + 8 * mmax * 16; }
-    __host__ __device__ int bytes() const { return off_mask() + nmax * wmax * 4; }
+    __host__ __device__ int end_base() const { return off_mask() + nmax * wmax * 4; }
+    __host__ __device__ int off_gain() const { return r16(end_base()); }
+    __host__ __device__ int bytes() const { return learned ? off_gain() + r16((mmax + PC_MAXC) * 4) : end_base(); }
 };
 
 // Work queues: class c owns order[qoff[c] .. qoff[c]+qcnt[c]) and counter
@@ -74,9 +63,10 @@ struct Queues {
     int qoff[4];
     int qcnt[4];
     int* counter;
+    int lowest;  // smallest class index a workgroup may steal from (its own index: no stealing)
 };
 
-template <int BLOCK, bool HB>
+template <int BLOCK, bool HB, bool PC>
 __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, RunParams P, Queues Q, int cls,
                                                                  LdsLayout L) {
     extern __shared__ float4 lds_dyn[];
@@ -88,10 +78,15 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
     float4* s_fng = reinterpret_cast<float4*>(lds + L.off_fng());
     unsigned* s_mask = reinterpret_cast<unsigned*>(lds + L.off_mask());
     float4* s_facc = reinterpret_cast<float4*>(lds + L.off_facc());
+    float* s_gain = reinterpret_cast<float*>(lds + L.off_gain());
+    const bool learned = PC && L.learned && P.pc_mode == 2;
     constexpr int NW = BLOCK / 64;
     __shared__ float s_red[4][NW];
     __shared__ double s_redd[NW];
     __shared__ int s_q;
+    // preconditioner: per-wave partial sums and per-component totals (PC_NSUM values each)
+    __shared__ float s_cred[PC ? NW : 1][PC ? PC_MAXC : 1][PC_NSUM];
+    __shared__ float s_csum[PC ? PC_MAXC : 1][PC_NSUM];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -101,10 +96,11 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
     for (;;) {
         if (tid == 0) {
             int q = -1;
-            while (cc >= 0) {
+            while (cc >= Q.lowest) {
                 const int k = atomicAdd(Q.counter + cc, 1);
                 if (k < Q.qcnt[cc]) {
                     q = Q.order[Q.qoff[cc] + k];
+                    if (P.chunk > 0 && B.status[q] != ST_RUNNING) continue;  // finished in an earlier chunk
                     break;
                 }
                 cc--;
@@ -130,6 +126,8 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
         const bool hasAtom = tid < N;
         const bool hasNode = tid < M;
         float4 plq = hasAtom ? B.PLQs[I.a0 + tid] : make_float4(0, 0, 0, 0);
+        const int myc = (PC && hasAtom) ? B.comp[I.a0 + tid] : 0;
+        const int nc = PC ? B.ncomp[s] : 0;
         int4 ng = make_int4(-1, -1, -1, -1);
         float4 apar = make_float4(0, 0, 0, 0), bL = apar, bK = apar, ksp = apar, kpp = apar;
         if (hasNode) {
@@ -161,13 +159,39 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
         fs.dt = P.fire.dt_max;
         fs.damping = P.fire.damp_max;
         fs.lastNeg = 0;
+        int step = 0;
+        if (P.chunk > 0) {  // resume the suspended state of this system
+            const float4 fst = B.fire[s];
+            fs.dt = fst.x;
+            fs.damping = fst.y;
+            fs.lastNeg = (int)fst.z;
+            step = B.nsteps[s];
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const int v = tid + r * BLOCK;
+                if (v < V) vel[r] = xyz(B.avel[I.v0 + v]);
+            }
+        }
+        if (learned) {
+            for (int k = tid; k < M; k += BLOCK) s_gain[k] = B.gnode[I.n0 + k];
+            if (tid < PC_MAXC) s_gain[M + tid] = B.gcomp[s * PC_MAXC + tid];
+        }
+        const int step_end = (P.chunk > 0) ? step + P.chunk : 0x7fffffff;
+        bool suspended = false;
         __syncthreads();
 
-        int step = 0;
         int status = ST_RUNNING;
         float fmax2_last = 0.f;
         V3 fv[2];
         for (;;) {
+            if (step >= step_end) {  // chunk done: suspend (state written back below)
+                suspended = true;
+                break;
+            }
+            if (B.snap && step % B.snap_every == 0 && step / B.snap_every < B.nsnap) {
+                float4* dst = B.snap + (size_t)(step / B.snap_every) * B.ntotV + I.v0;
+                for (int v = tid; v < V; v += BLOCK) dst[v] = s_pos[v];
+            }
             // ================= A: forces (no energies inside the step loop)
             // Half-ring pair schedule (Newton's third law): thread i takes the
             // partners j = (i+k) mod N, k = 1..N/2, so every unordered pair is
@@ -284,8 +308,19 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
                         const V3 p = xyz(s_pos[v]);
                         f -= p * dot(p, f);
                     }
+                    fm2 = fmaxf(fm2, dot(f, f));  // convergence always on the exact force
+                    fv[r] = f;
+                }
+            }
+            if (PC)
+                precondition<BLOCK>(P, s_pos, s_fnb, s_fself, s_fng, &s_cred[0][0][0], &s_csum[0][0], s_gain, learned, N,
+                                    M, nc, myc, hasAtom, hasNode, ng, bk[0], fv[0], [] {});
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const int v = tid + r * BLOCK;
+                if (v < V) {
+                    V3 f = fv[r];
                     const float f2 = dot(f, f);
-                    fm2 = fmaxf(fm2, f2);  // convergence always on the exact force
                     if (P.Flimit > 0.f && f2 > P.Flimit * P.Flimit) f = f * (P.Flimit / sqrtf(f2));
                     fv[r] = f;
                     vf += dot(vel[r], f);
@@ -402,6 +437,10 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
             __syncthreads();
         }
         (void)fmax2_last;
+        if (suspended && tid == 0) {
+            B.fire[s] = make_float4(fs.dt, fs.damping, (float)fs.lastNeg, 0.f);
+            B.nsteps[s] = step;
+        }
         // ---- write back the final state of this system
 #pragma unroll
         for (int r = 0; r < 2; r++) {
@@ -409,34 +448,41 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
             if (v < V) {
                 B.atoms[I.v0 + v] = s_pos[v];
                 B.avel[I.v0 + v] = make_float4(vel[r].x, vel[r].y, vel[r].z, 0.f);
-                B.aforce[I.v0 + v] = make_float4(fv[r].x, fv[r].y, fv[r].z, 0.f);
+                const V3 fo = (PC && v < N) ? xyz(s_fnb[v]) : fv[r];
+                B.aforce[I.v0 + v] = make_float4(fo.x, fo.y, fo.z, 0.f);
             }
         }
         __syncthreads();
     }
 }
 
-#define FC_INST(BL, HB) \
-    template __global__ void relax_persistent_kernel<BL, HB>(BatchDev, RunParams, Queues, int, LdsLayout);
-FC_INST(64, false)
-FC_INST(64, true)
-FC_INST(128, false)
-FC_INST(128, true)
-FC_INST(192, false)
-FC_INST(192, true)
-FC_INST(256, false)
-FC_INST(256, true)
+#define FC_INST(BL, HB, PCF) \
+    template __global__ void relax_persistent_kernel<BL, HB, PCF>(BatchDev, RunParams, Queues, int, LdsLayout);
+#define FC_INST2(BL)          \
+    FC_INST(BL, false, false) \
+    FC_INST(BL, true, false)  \
+    FC_INST(BL, false, true)  \
+    FC_INST(BL, true, true)
+FC_INST2(64)
+FC_INST2(128)
+FC_INST2(192)
+FC_INST2(256)
+#undef FC_INST2
 #undef FC_INST
 
-template <bool HB>
+template <bool HB, bool PCF>
 static const void* kernel_ptr(int block) {
     switch (block) {
-        case 64: return (const void*)&relax_persistent_kernel<64, HB>;
-        case 128: return (const void*)&relax_persistent_kernel<128, HB>;
-        case 192: return (const void*)&relax_persistent_kernel<192, HB>;
-        case 256: return (const void*)&relax_persistent_kernel<256, HB>;
+        case 64: return (const void*)&relax_persistent_kernel<64, HB, PCF>;
+        case 128: return (const void*)&relax_persistent_kernel<128, HB, PCF>;
+        case 192: return (const void*)&relax_persistent_kernel<192, HB, PCF>;
+        case 256: return (const void*)&relax_persistent_kernel<256, HB, PCF>;
         default: return nullptr;
     }
+}
+static const void* kernel_ptr(int block, int hb, int pc) {
+    if (pc) return hb ? kernel_ptr<true, true>(block) : kernel_ptr<false, true>(block);
+    return hb ? kernel_ptr<true, false>(block) : kernel_ptr<false, false>(block);
 }
 
 }  // namespace fcd
@@ -445,8 +491,8 @@ static const void* kernel_ptr(int block) {
 extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups, size_t lds_bytes, hipStream_t stream,
                                                  const fcd::BatchDev* B, const fcd::RunParams* P, const int* order,
                                                  const int* qinfo, int* counters, int cls, int nmax, int mmax,
-                                                 int wmax) {
-    fcd::LdsLayout L{nmax, mmax, wmax, block / 64};
+                                                 int wmax, int steal) {
+    fcd::LdsLayout L{nmax, mmax, wmax, block / 64, P->pc_mode == 2 ? 1 : 0};
     fcd::Queues Q;
     Q.order = order;
     for (int c = 0; c < 4; c++) {
@@ -454,19 +500,20 @@ extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups,
         Q.qcnt[c] = qinfo[4 + c];
     }
     Q.counter = counters;
-    const void* k = hb ? fcd::kernel_ptr<true>(block) : fcd::kernel_ptr<false>(block);
+    Q.lowest = steal ? 0 : cls;
+    const void* k = fcd::kernel_ptr(block, hb, P->pc_mode != 0);
     if (!k) return hipErrorInvalidValue;
     void* args[] = {(void*)B, (void*)P, (void*)&Q, (void*)&cls, (void*)&L};
     return hipLaunchKernel(k, dim3(ngroups), dim3(block), args, lds_bytes, stream);
 }
 
-extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block) {
-    fcd::LdsLayout L{nmax, mmax, wmax, block / 64};
+extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block, int learned) {
+    fcd::LdsLayout L{nmax, mmax, wmax, block / 64, learned};
     return L.bytes();
 }
 
-extern "C" hipError_t fc_persistent_occupancy(int block, int hb, size_t lds, int* nblocks) {
-    const void* k = hb ? fcd::kernel_ptr<true>(block) : fcd::kernel_ptr<false>(block);
+extern "C" hipError_t fc_persistent_occupancy(int block, int hb, int pc, size_t lds, int* nblocks) {
+    const void* k = fcd::kernel_ptr(block, hb, pc);
     if (!k) return hipErrorInvalidValue;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(nblocks, k, block, lds);
 }

@@ -6,6 +6,7 @@
 // full per-system FIRE controller on the device.  They are the parity
 // reference for the persistent kernel and the HBM-bound form of the path.
 #include "kernels.h"
+#include "precond.h"
 
 namespace fcd {
 
@@ -249,6 +250,51 @@ __global__ void k_zero_f4(float4* a, int n) {
     if (i < n) a[i] = make_float4(0, 0, 0, 0);
 }
 
+
+// Preconditioned force of every system from the exact forces in B.aforce
+// (one 256-thread block per system; systems must have N <= 256 and M <= 256).
+// gains: optional learned gains, per node at [n0 + k] of gnode and per
+// system component at [s * PC_MAXC + c] of gcomp; nullptr = P.pc_gstar/grigid.
+__global__ void __launch_bounds__(256) k_precond(BatchDev B, RunParams P, float4* out, const float* gnode,
+                                                 const float* gcomp) {
+    constexpr int BLOCK = 256;
+    constexpr int NW = BLOCK / 64;
+    __shared__ float4 s_pos[BLOCK];
+    __shared__ float4 s_fnb[BLOCK];
+    __shared__ float4 s_fself[BLOCK];
+    __shared__ float4 s_fng[4 * BLOCK];
+    __shared__ float s_cred[NW][PC_MAXC][PC_NSUM];
+    __shared__ float s_csum[PC_MAXC][PC_NSUM];
+    __shared__ float s_gain[BLOCK + PC_MAXC];
+    const int s = blockIdx.x;
+    const SysInfo I = B.info[s];
+    const int N = I.natoms, M = I.nnode;
+    if (N > BLOCK || M > BLOCK) return;  // uniform per block
+    const int tid = threadIdx.x;
+    const bool hasAtom = tid < N, hasNode = tid < M;
+    if (hasAtom) s_pos[tid] = B.atoms[I.v0 + tid];
+    const bool learned = gnode != nullptr;
+    if (learned) {
+        if (hasNode) s_gain[tid] = gnode[I.n0 + tid];
+        if (tid < PC_MAXC) s_gain[M + tid] = gcomp[s * PC_MAXC + tid];
+    }
+    int4 ng = make_int4(-1, -1, -1, -1), bk0 = ng;
+    if (hasNode) ng = B.neighs[I.a0 + tid];
+    if (hasAtom) {
+        const int4 b = B.bkNeighs[I.v0 + tid];
+        const int base = I.n0 * 8;
+        bk0 = make_int4(b.x >= 0 ? b.x - base : -1, b.y >= 0 ? b.y - base : -1, b.z >= 0 ? b.z - base : -1,
+                        b.w >= 0 ? b.w - base : -1);
+    }
+    V3 f = hasAtom ? xyz(B.aforce[I.v0 + tid]) : zero3();
+    const int myc = hasAtom ? B.comp[I.a0 + tid] : 0;
+    __syncthreads();
+    precondition<BLOCK>(P, s_pos, s_fnb, s_fself, s_fng, &s_cred[0][0][0], &s_csum[0][0], s_gain, learned, N, M,
+                        B.ncomp[s], myc, hasAtom, hasNode, ng, bk0, f, [] {});
+    if (hasAtom) out[I.v0 + tid] = make_float4(f.x, f.y, f.z, 0.f);
+    for (int v = N + tid; v < N + M; v += BLOCK) out[I.v0 + v] = B.aforce[I.v0 + v];
+}
+
 }  // namespace fcd
 
 extern "C" hipError_t fc_launch_split_step(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P,
@@ -268,5 +314,11 @@ extern "C" hipError_t fc_launch_reset(hipStream_t st, const fcd::BatchDev* B, co
 extern "C" hipError_t fc_launch_zero_f4(hipStream_t st, float4* a, int n) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(fcd::k_zero_f4, dim3((n + 255) / 256), dim3(256), 0, st, a, n);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t fc_launch_precond(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P, float4* out,
+                                        const float* gnode, const float* gcomp) {
+    hipLaunchKernelGGL(fcd::k_precond, dim3(B->nsys), dim3(256), 0, st, *B, *P, out, gnode, gcomp);
     return hipGetLastError();
 }

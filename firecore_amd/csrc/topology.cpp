@@ -1,6 +1,7 @@
 // Topology bookkeeping: back-neighbours and non-bonded exclusions.
 #include "fc_host.h"
 
+#include <algorithm>
 #include <cstring>
 
 namespace fc {
@@ -91,6 +92,35 @@ bool rotate_exclusions(int natoms, const std::vector<uint32_t>& mask, int W, std
         }
     }
     return sym;
+}
+
+// Connected components of the bonded graph given by the neighbour lists
+// (bonds listed on either end).  Component indices are assigned in order of
+// the lowest atom index; indices >= maxc are folded into maxc-1 (the
+// molecule-level projection then treats those molecules as one rigid group).
+// Returns the number of distinct (folded) indices.
+int connected_components(int natoms, const int* neighs, int maxc, int* comp) {
+    std::vector<int> parent(natoms);
+    for (int i = 0; i < natoms; i++) parent[i] = i;
+    auto find = [&](int x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    for (int i = 0; i < natoms; i++)
+        for (int k = 0; k < 4; k++) {
+            const int j = neighs[i * 4 + k];
+            if (j < 0 || j >= natoms) continue;
+            int a = find(i), b = find(j);
+            if (a != b) parent[std::max(a, b)] = std::min(a, b);
+        }
+    std::vector<int> id(natoms, -1);
+    int nc = 0;
+    for (int i = 0; i < natoms; i++) {
+        const int r = find(i);
+        if (id[r] < 0) id[r] = nc++;
+        comp[i] = std::min(id[r], maxc - 1);
+    }
+    return std::min(nc, maxc);
 }
 
 }  // namespace fc

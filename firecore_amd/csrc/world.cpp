@@ -6,6 +6,8 @@
 // for pi rows, datatypes_utils.h:19 copy_add).
 #include "world.h"
 
+#include "precond.h"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -15,12 +17,14 @@
 extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups, size_t lds_bytes, hipStream_t stream,
                                                  const fcd::BatchDev* B, const fcd::RunParams* P, const int* order,
                                                  const int* qinfo, int* counters, int cls, int nmax, int mmax,
-                                                 int wmax);
-extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block);
+                                                 int wmax, int steal);
+extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block, int learned);
 extern "C" hipError_t fc_launch_split_step(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P,
                                            const int* node_sys, int ntot_nodes, int maxN, int do_move, int step);
 extern "C" hipError_t fc_launch_reset(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P);
-extern "C" hipError_t fc_persistent_occupancy(int block, int hb, size_t lds, int* nblocks);
+extern "C" hipError_t fc_launch_precond(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P, float4* out,
+                                        const float* gnode, const float* gcomp);
+extern "C" hipError_t fc_persistent_occupancy(int block, int hb, int pc, size_t lds, int* nblocks);
 
 namespace fcw {
 
@@ -54,6 +58,17 @@ void DevArr<T>::download(std::vector<T>& h, hipStream_t s) const {
     size_t cnt = std::min(h.size(), n);
     if (cnt) hip_ok(hipMemcpyAsync(h.data(), p, cnt * sizeof(T), hipMemcpyDeviceToHost, s), "download");
 }
+
+// explicit instantiations (the other translation units use these)
+template struct DevArr<float4>;
+template struct DevArr<int4>;
+template struct DevArr<int>;
+template struct DevArr<unsigned>;
+template struct DevArr<unsigned short>;
+template struct DevArr<float>;
+template struct DevArr<double>;
+template struct DevArr<fcd::SysInfo>;
+template struct DevArr<fcd::PairCorrDev>;
 
 World& world() {
     static World W;
@@ -137,6 +152,8 @@ void World::build_packed() {
     BLs = BKs = Ksp = Kpp = MMpars;
     node_sys.assign(ntotM, 0);
     elem.assign(ntotA, 0);
+    comp.assign(ntotA, 0);
+    ncomp.assign(nSys, 1);
     lvecs.assign((size_t)nSys * 3, f4(0, 0, 0, 0));
     ilvecs.assign((size_t)nSys * 3, f4(0, 0, 0, 0));
     pbcshifts.assign((size_t)nSys, f4(0, 0, 0, 0));
@@ -200,6 +217,7 @@ void World::pack_system(int s, bool bParams, bool bForces, bool bVel, bool blvec
         Kpp[I.n0 + k] = P4(m.Kpp);
         node_sys[I.n0 + k] = s;
     }
+    ncomp[s] = fc::connected_components(N, m.neighs.data(), fcd::PC_MAXC, comp.data() + I.a0);
     std::vector<int> bk((size_t)N * 4);
     if (!fc::make_back_neighs(N, M, m.neighs.data(), bk.data()))
         std::fprintf(stderr, "[libMMFFmulti_lib] system %d: an atom has more than 4 back-neighbours\n", s);
@@ -278,6 +296,9 @@ bool World::upload_all(bool bParams, bool bForces, bool bVel) {
         d_corr.upload(corr, stream);
         d_info.upload(info, stream);
         d_node_sys.upload(node_sys, stream);
+        d_comp.upload(comp, stream);
+        d_ncomp.upload(ncomp, stream);
+        d_elem.upload(elem, stream);
         d_fneigh.alloc(std::max<size_t>((size_t)ntotM * 8, 1));
         d_ebond.alloc(std::max<size_t>((size_t)ntotM, 1));
         d_enb.alloc(std::max<size_t>((size_t)ntotA, 1));
@@ -368,6 +389,15 @@ fcd::BatchDev World::batch_dev() {
     B.Esys = d_Esys.p;
     B.Fmax = d_Fmax.p;
     B.fire = d_fire.p;
+    B.comp = d_comp.p;
+    B.ncomp = d_ncomp.p;
+    B.elem = d_elem.p;
+    B.gnode = d_gnode.p;
+    B.gcomp = d_gcomp.p;
+    B.snap = (snap_on && d_snap.p) ? d_snap.p : nullptr;
+    B.snap_every = std::max(snap_every, 1);
+    B.nsnap = nsnap;
+    B.ntotV = ntotV;
     return B;
 }
 
@@ -392,6 +422,20 @@ fcd::RunParams World::run_params(double Fconv, int nstep_max) {
     P.conv_mode = conv_mode;
     P.Flimit = Flimit;
     P.nstep_max = nstep_max;
+    P.pc_mode = pc_mode;
+    P.pc_gstar = pc_gstar;
+    P.pc_grigid = pc_grigid;
+    P.pc_every = std::max(pc_every, 1);
+    if (d_mlpW.p) {
+        P.mlp.W1 = d_mlpW.p;
+        P.mlp.W2 = d_mlpW.p + 64 * 16;
+        P.mlp.W3 = d_mlpW.p + 64 * 16 + 64 * 128;
+        P.mlp.b1 = d_mlpF.p;
+        P.mlp.b2 = d_mlpF.p + 64;
+        P.mlp.b3 = d_mlpF.p + 128;
+        P.mlp.W4 = d_mlpF.p + 192;
+        P.mlp.b4 = d_mlpF.p + 192 + 128;
+    }
     return P;
 }
 
@@ -448,7 +492,8 @@ void World::plan_classes() {
         C.nmax = nrun;
         C.mmax = mrun;
         C.wmax = (C.nmax + 31) / 32;
-        C.lds = (size_t)fc_lds_bytes(std::max(C.nmax, 1), std::max(C.mmax, 1), std::max(C.wmax, 1), C.block);
+        C.lds = (size_t)fc_lds_bytes(std::max(C.nmax, 1), std::max(C.mmax, 1), std::max(C.wmax, 1), C.block,
+                                     pc_mode == 2 ? 1 : 0);
         auto cost = [&](int s) {
             const double N = info[s].natoms;
             const double st = (nsteps.size() == (size_t)nSys && nsteps[s] > 0) ? nsteps[s] + 1.0 : 1000.0 + 20.0 * N;
@@ -472,13 +517,16 @@ void World::plan_classes() {
         C.cls = c;
         if (C.order.empty()) continue;
         int nb = 1;
-        fc_persistent_occupancy(C.block, hb_batch ? 1 : 0, C.lds, &nb);
+        fc_persistent_occupancy(C.block, hb_batch ? 1 : 0, pc_mode != 0 ? 1 : 0, C.lds, &nb);
         nb = std::max(nb, 1);
         C.occ = nb;
         int g = (int)std::lround((double)ncu * nb * C.work / std::max(wtot, 1e-30));
+        if (sched_mode == 1) g = ncu * nb;  // every class may fill the device; the dispatcher interleaves them
         C.ngroups = std::max(1, std::min(g, (int)C.order.size()));
         classes.push_back(C);
     }
+    // launch the most expensive class first
+    std::stable_sort(classes.begin(), classes.end(), [](const SizeClass& a, const SizeClass& b) { return a.work > b.work; });
     qinfo.assign(8, 0);
     for (int c = 0; c < 4; c++) {
         qinfo[c] = qoff[c];
@@ -489,7 +537,9 @@ void World::plan_classes() {
 }
 
 int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
+    last_chunks = 0;
     if (!dev_ready || nSys == 0) return -1;
+    if (pc_mode == 2) return relax_learned(nstep_max, Fconv, restore_start);
     if (!ring_ok) {
         std::fprintf(stderr, "[libMMFFmulti_lib] asymmetric exclusions: relaxing with the split kernels\n");
         return relax_split(nstep_max, Fconv, restore_start);
@@ -516,7 +566,8 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
         hipEventCreate(&ev);
         hipEventRecord(es, st);
         hipError_t err = fc_launch_relax_persistent(C.block, hb_batch ? 1 : 0, C.ngroups, C.lds, st, &B, &P, d_order.p,
-                                                    qinfo.data(), d_queue.p, C.cls, C.nmax, C.mmax, C.wmax);
+                                                    qinfo.data(), d_queue.p, C.cls, C.nmax, C.mmax, C.wmax,
+                                                    sched_mode == 0 ? 1 : 0);
         if (!hip_ok(err, "relax_persistent launch")) return -1;
         hipEventRecord(ev, st);
         hipStreamWaitEvent(stream, ev, 0);
@@ -595,6 +646,69 @@ bool World::eval_forces() {
     return hip_ok(hipStreamSynchronize(stream), "eval_forces sync");
 }
 
+// Exact forces (split kernels) followed by the preconditioned force of the
+// current gains into pcforces (host mirror, float4 per vec).
+bool World::eval_precond() {
+    if (!eval_forces()) return false;
+    if (maxN > 256 || maxM > 256) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] eval_precond: systems above 256 atoms are not supported\n");
+        return false;
+    }
+    fcd::BatchDev B = batch_dev();
+    fcd::RunParams P = run_params(0.0, 0);
+    d_pcforce.alloc(std::max(ntotV, 1));
+    if (!hip_ok(fc_launch_precond(stream, &B, &P, d_pcforce.p, nullptr, nullptr), "precond")) return false;
+    pcforces.resize(ntotV);
+    d_pcforce.download(pcforces, stream);
+    d_aforce.download(aforces, stream);
+    return hip_ok(hipStreamSynchronize(stream), "eval_precond sync");
+}
+
+static unsigned short bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+// Install proposal-network parameters (FP32, packing order of mlp_theta):
+// dense weights rounded to bf16 for the matrix cores, the rest kept FP32.
+bool World::set_mlp(const float* theta, int n) {
+    const int NP = 64 * 16 + 64 + 64 * 128 + 64 + 64 * 128 + 64 + 2 * 64 + 2;
+    if (n != NP) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] proposal weights: expected %d values, got %d\n", NP, n);
+        return false;
+    }
+    for (int i = 0; i < n; i++)
+        if (!std::isfinite(theta[i])) {
+            std::fprintf(stderr, "[libMMFFmulti_lib] proposal weights: non-finite value at %d\n", i);
+            return false;
+        }
+    if (!ensure_device()) return false;
+    mlp_theta.assign(theta, theta + n);
+    const float* W1 = theta;
+    const float* b1 = W1 + 64 * 16;
+    const float* W2 = b1 + 64;
+    const float* b2 = W2 + 64 * 128;
+    const float* W3 = b2 + 64;
+    const float* b3 = W3 + 64 * 128;
+    const float* W4 = b3 + 64;
+    const float* b4 = W4 + 128;
+    std::vector<unsigned short> wb;
+    for (int i = 0; i < 64 * 16; i++) wb.push_back(bf16_rne(W1[i]));
+    for (int i = 0; i < 64 * 128; i++) wb.push_back(bf16_rne(W2[i]));
+    for (int i = 0; i < 64 * 128; i++) wb.push_back(bf16_rne(W3[i]));
+    std::vector<float> fb;
+    fb.insert(fb.end(), b1, b1 + 64);
+    fb.insert(fb.end(), b2, b2 + 64);
+    fb.insert(fb.end(), b3, b3 + 64);
+    fb.insert(fb.end(), W4, W4 + 128);
+    fb.insert(fb.end(), b4, b4 + 2);
+    d_mlpW.upload(wb, stream);
+    d_mlpF.upload(fb, stream);
+    return hip_ok(hipStreamSynchronize(stream), "set_mlp");
+}
+
 bool World::sync() { return dev_ready ? hip_ok(hipStreamSynchronize(stream), "sync") : true; }
 
 void World::register_buffers() {
@@ -636,6 +750,9 @@ void World::register_buffers() {
     ibuffers["gpu_elem"] = elem.data();
     fbuffers["gpu_PLQs"] = (float*)PLQs.data();
     fbuffers["sys_Fmax"] = Fmax.data();
+    pcforces.assign(std::max(ntotV, 1), make_float4(0.f, 0.f, 0.f, 0.f));
+    fbuffers["gpu_pcforces"] = (float*)pcforces.data();
+    ibuffers["gpu_comp"] = comp.data();
     buffers["sys_E"] = Esys.data();
 }
 

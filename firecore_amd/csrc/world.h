@@ -46,6 +46,40 @@ struct World {
     fcd::FireParams fp{0.05f, 0.005f, 0.1f, 1.1f, 0.5f, 0.8f, 0.0f, 3};
     int conv_mode = fcd::CONV_FMAX;
     float Flimit = 0.f;  // per-vec force clamp for the dynamics (0 = off)
+    // FIRE force preconditioner (precond.h): mode 0 off, 1 constant gains, 2 learned gains
+    int pc_mode = 0;
+    float pc_gstar = 3.f, pc_grigid = 63.f;
+    int pc_every = 8;
+    // proposal network parameters: FP32 master copy in the packing order
+    // W1[64][16] b1[64] W2[64][128] b2[64] W3[64][128] b3[64] W4[2][64] b4[2]
+    std::vector<float> mlp_theta;
+    DevArr<unsigned short> d_mlpW;  // bf16 W1 | W2 | W3
+    DevArr<float> d_mlpF;           // fp32 b1 | b2 | b3 | W4 | b4
+    bool set_mlp(const float* theta, int n);
+    // ---- proposal training (train.cpp)
+    DevArr<float4> d_snap, d_x0, d_F0;
+    int snap_every = 0, nsnap = 0;
+    bool snap_on = false;
+    DevArr<double> d_E0;
+    DevArr<unsigned short> d_X, d_hs;
+    DevArr<float> d_z, d_out, d_dout, d_gnode, d_gcomp, d_loss, d_gpart, d_grad, d_theta, d_am, d_av, d_lossmean;
+    DevArr<int> d_snapidx;
+    int adam_t = 0;
+    bool theta_dev = false;  // d_theta holds the current parameters
+    bool record_snapshots(int every, int n);
+    bool load_snapshots(const std::vector<int>& idx);
+    bool prop_alloc();
+    bool prop_prepare();
+    bool prop_forward();
+    bool prop_loss_grad(float eta, float dmax);
+    bool prop_backward();
+    bool prop_adam(float lr);
+    bool download_theta();
+    bool restore_positions();
+    int relax_learned(int nstep_max, double Fconv, bool restore_start);
+    DevArr<int> d_count;
+    bool use_graphs = true;
+    int last_chunks = 0;
 
     // ---- systems (host, double) and the template system (system 0)
     std::vector<fc::MolSys> mols;
@@ -55,12 +89,13 @@ struct World {
     std::vector<fcd::SysInfo> info;
 
     // ---- host mirrors in the reference's packed float layout
+    std::vector<float4> pcforces;  // preconditioned forces of the last eval_precond
     std::vector<float4> atoms, aforces, avel, REQs, PLQs, MMpars, BLs, BKs, Ksp, Kpp, constr, lvecs, ilvecs, pbcshifts;
     std::vector<int4> neighs, neighCell, bkNeighs;
     std::vector<unsigned> excl, excl_rot;
     bool ring_ok = true;
     std::vector<fcd::PairCorrDev> corr;
-    std::vector<int> node_sys, elem;
+    std::vector<int> node_sys, elem, comp, ncomp;
     std::vector<int> nsteps, status;
     std::vector<double> Esys;
     std::vector<float> Fmax;
@@ -78,19 +113,23 @@ struct World {
     bool dev_ready = false;
     hipStream_t stream = nullptr;
     std::vector<hipStream_t> class_streams;
+    DevArr<float4> d_pcforce;
     DevArr<float4> d_atoms, d_atoms0, d_aforce, d_avel, d_REQs, d_PLQs, d_apars, d_bLs, d_bKs, d_Ksp, d_Kpp, d_fneigh,
         d_fire, d_grid;
     DevArr<int4> d_neighs, d_bkNeighs;
     DevArr<unsigned> d_excl, d_excl_rot;
     DevArr<fcd::PairCorrDev> d_corr;
     DevArr<fcd::SysInfo> d_info;
-    DevArr<int> d_node_sys, d_nsteps, d_status, d_order, d_queue;
+    DevArr<int> d_node_sys, d_nsteps, d_status, d_order, d_queue, d_comp, d_ncomp, d_elem;
     DevArr<double> d_Esys, d_enb;
     DevArr<float> d_Fmax, d_ebond;
     std::vector<SizeClass> classes;
     std::vector<int> qinfo, sys_class;
     bool hb_batch = false;
     bool autotune = true;
+    // 0: device slots split between classes by estimated time, idle workgroups steal
+    //    from smaller classes; 1: every class launched at full device width, no stealing
+    int sched_mode = 1;
     double last_ms = 0;
 
     // ---- buffer registry (libUtils.h:16-66)
@@ -116,6 +155,7 @@ struct World {
     int relax_persistent(int nstep_max, double Fconv, bool restore_start);
     int relax_split(int nstep_max, double Fconv, bool restore_start);
     bool eval_forces();
+    bool eval_precond();
     bool sync();
 };
 

@@ -314,6 +314,136 @@ def set_force_limit(Flimit):
     L.set_force_limit(Flimit)
 
 
+def eval_precond():
+    """Exact forces (Batch.forces) and preconditioned forces (returned, (ntotV,4))."""
+    L = lib()
+    L.eval_precond.restype = c_int
+    if L.eval_precond() != 0:
+        raise RuntimeError("eval_precond failed")
+    nt = batch_sizes()["ntotV"]
+    return fview("gpu_pcforces", nt * 4).reshape(nt, 4).copy()
+
+
+def set_sched(mode):
+    L = lib()
+    L.set_sched.argtypes = [c_int]
+    L.set_sched.restype = c_int
+    r = L.set_sched(mode)
+    if r < 0:
+        raise ValueError("set_sched: invalid mode")
+    return r
+
+
+def set_precond(mode, gstar=3.0, grigid=63.0):
+    L = lib()
+    L.set_precond.argtypes = [c_int, c_double, c_double]
+    L.set_precond.restype = c_int
+    if L.set_precond(mode, gstar, grigid) != 0:
+        raise ValueError("set_precond: invalid arguments")
+
+
 def set_opt(dt_max=0.05, dt_min=0.005, damp_max=0.1, finc=1.1, fdec=0.5, falpha=0.8, minLastNeg=3,
             cvf_min=-0.1, cvf_max=0.1):
     lib().set_opt(dt_max, dt_min, damp_max, finc, fdec, falpha, minLastNeg, cvf_min, cvf_max)
+
+
+# ---- proposal network -------------------------------------------------------
+def proposal_nparam():
+    L = lib()
+    L.get_proposal_nparam.restype = c_int
+    return L.get_proposal_nparam()
+
+
+def set_proposal_weights(theta):
+    L = lib()
+    th = np.ascontiguousarray(theta, np.float32)
+    L.set_proposal_weights.argtypes = [ctypes.c_void_p, c_int]
+    L.set_proposal_weights.restype = c_int
+    if L.set_proposal_weights(th.ctypes.data, th.size) != 0:
+        raise ValueError("set_proposal_weights failed")
+
+
+def get_proposal_weights():
+    L = lib()
+    n = proposal_nparam()
+    th = np.zeros(n, np.float32)
+    L.get_proposal_weights.argtypes = [ctypes.c_void_p, c_int]
+    L.get_proposal_weights.restype = c_int
+    if L.get_proposal_weights(th.ctypes.data, n) != 0:
+        raise RuntimeError("get_proposal_weights failed")
+    return th
+
+
+def set_proposal_every(every):
+    L = lib()
+    L.set_proposal_every.argtypes = [c_int]
+    L.set_proposal_every.restype = c_int
+    if L.set_proposal_every(every) != 0:
+        raise ValueError("set_proposal_every: invalid period")
+
+
+def proposal_forward():
+    """(outputs (ntotA,2), features (ntotA,16)) at the current positions."""
+    L = lib()
+    nA = batch_sizes()["ntotA"]
+    out = np.zeros((nA, 2), np.float32)
+    feats = np.zeros((nA, 16), np.float32)
+    L.proposal_forward.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.proposal_forward.restype = c_int
+    if L.proposal_forward(out.ctypes.data, feats.ctypes.data) != 0:
+        raise RuntimeError("proposal_forward failed")
+    return out, feats
+
+
+def proposal_loss_grad(eta, dmax):
+    L = lib()
+    s = batch_sizes()
+    dout = np.zeros((s["ntotA"], 2), np.float32)
+    loss = np.zeros(s["nsys"], np.float32)
+    L.proposal_loss_grad.argtypes = [c_double, c_double, ctypes.c_void_p, ctypes.c_void_p]
+    L.proposal_loss_grad.restype = c_int
+    if L.proposal_loss_grad(eta, dmax, dout.ctypes.data, loss.ctypes.data) != 0:
+        raise RuntimeError("proposal_loss_grad failed")
+    return dout, loss
+
+
+def proposal_backward(dout):
+    L = lib()
+    d = np.ascontiguousarray(dout, np.float32)
+    g = np.zeros(proposal_nparam(), np.float32)
+    L.proposal_backward.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.proposal_backward.restype = c_int
+    if L.proposal_backward(d.ctypes.data, g.ctypes.data) != 0:
+        raise RuntimeError("proposal_backward failed")
+    return g
+
+
+def record_snapshots(every, n):
+    L = lib()
+    L.record_snapshots.argtypes = [c_int, c_int]
+    L.record_snapshots.restype = c_int
+    if L.record_snapshots(every, n) != 0:
+        raise RuntimeError("record_snapshots failed")
+
+
+def train_proposal(iters, lr, eta, dmax, seed=0):
+    L = lib()
+    hist = np.zeros(max(iters, 1), np.float32)
+    L.train_proposal.argtypes = [c_int, c_double, c_double, c_double, ctypes.c_longlong, ctypes.c_void_p]
+    L.train_proposal.restype = c_int
+    if L.train_proposal(iters, lr, eta, dmax, seed, hist.ctypes.data) != 0:
+        raise RuntimeError("train_proposal failed")
+    return hist[:iters]
+
+
+def last_chunks():
+    L = lib()
+    L.get_last_chunks.restype = c_int
+    return L.get_last_chunks()
+
+
+def set_graphs(on):
+    L = lib()
+    L.set_graphs.argtypes = [c_int]
+    L.set_graphs.restype = c_int
+    return L.set_graphs(1 if on else 0)

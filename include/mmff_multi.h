@@ -193,6 +193,44 @@ void set_conv_mode(int mode);
  * behaviour; the reference GPU update kernel uses 10 eV/A,
  * relax_multi.cl:980-983).  Convergence is tested on the unclamped force. */
 void set_force_limit(double Flimit);
+/* FIRE force preconditioner (no counterpart in the reference, whose GPU and
+ * CPU paths integrate the raw force): 0 off, 1 rigid-group projections with
+ * constant gains (gstar per bonded star, grigid per molecule), 2 gains from
+ * the proposal network (ceilings gstar, grigid).  0 on success, -1 invalid. */
+int set_precond(int mode, double gstar, double grigid);
+/* Exact forces (gpu_aforces) and preconditioned forces (view gpu_pcforces)
+ * of every system at the current positions (no counterpart in the
+ * reference).  0 on success. */
+int eval_precond(void);
+/* ---- proposal network (no counterpart in the reference) ----
+ * A bonded-graph message-passing MLP (16 -> 64 -> 64 -> 64 -> 2, BF16 dense
+ * layers on the matrix cores) that predicts the preconditioner gains from
+ * per-atom invariants of the force, pi-orbital and substrate force.
+ * Parameters: FP32, packing W1[64][16] b1 W2[64][128] b2 W3[64][128] b3
+ * W4[2][64] b4[2].  All return 0 on success, -1 on error. */
+int get_proposal_nparam(void);
+int set_proposal_weights(const float* theta, int n);
+int get_proposal_weights(float* theta, int n);
+int set_proposal_every(int every);
+/* trajectory snapshots of the next relax() (every `every` steps, at most n) */
+int record_snapshots(int every, int n);
+int load_snapshots(const int* idx);
+/* forward pass at the current positions: out [ntotA][2], feats [ntotA][16] */
+int proposal_forward(float* out, float* feats);
+/* loss of one preconditioned step and its gradient w.r.t. the outputs */
+int proposal_loss_grad(double eta, double dmax, float* dout, float* loss);
+/* backward pass for given dloss/doutputs; grad = mean over systems */
+int proposal_backward(const float* dout, float* grad);
+/* Adam training on the recorded snapshots, one snapshot per system per step */
+int train_proposal(int iters, double lr, double eta, double dmax, long long seed, float* loss_hist);
+/* chunk iterations of the last learned relaxation */
+int get_last_chunks(void);
+/* 1 = replay the learned chunk loop from a captured HIP graph, 0 = direct launches */
+int set_graphs(int on);
+/* Persistent-path scheduling (no counterpart in the reference): 0 = slots
+ * split between size classes by estimated time with work stealing, 1 = every
+ * class at full device width, no stealing.  Returns the previous mode. */
+int set_sched(int mode);
 /* nSys, ntotA, ntotM, ntotV, maxN, maxM, grid nx, ny, nz */
 void get_batch_sizes(int* out9);
 /* grid pos0[3], step[3], alphaMorse; returns number of coefficients */

@@ -1,0 +1,276 @@
+"""Proposal network: FP64 oracle pins (CPU) and device parity (GPU).
+
+Pins of the oracle (oracle/mlp.py): its analytic backward pass against
+central finite differences in FP64, its bfloat16 rounding against torch's,
+rotation invariance of the features about the substrate normal, and the
+training-loss gradient (dE/dgain = -eta F(x').P(F0)) against finite
+differences of the FP64 force-field energy.  The device forward, backward and
+loss gradient are then compared with the oracle on the same inputs.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp
+from oracle import precond as pc
+from firecore_amd import native
+
+GSTAR, GRIGID = 3.0, 63.0
+
+
+def _chain_graph(N, rng):
+    neighs = -np.ones((N, 4), int)
+    for i in range(1, N):
+        j = int(rng.integers(0, i))
+        while (neighs[j] >= 0).sum() >= 4:
+            j = int(rng.integers(0, i))
+        neighs[i][list(neighs[i]).index(-1)] = j
+        neighs[j][list(neighs[j]).index(-1)] = i
+    return neighs
+
+
+def test_oracle_backward_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    N = 14
+    neighs = _chain_graph(N, rng)
+    nb = mlp.neighbour_lists(N, neighs)
+    x = rng.normal(size=(N, mlp.NIN))
+    th = mlp.init_params(1, head_scale=1.0)
+    w = rng.normal(size=(N, 2))
+    o, cache = mlp.forward(th, x, nb)
+    g = mlp.backward(th, cache, w, nb)
+    for p in rng.choice(mlp.NPARAM, 40, replace=False):
+        e = np.zeros(mlp.NPARAM)
+        e[p] = 1e-6
+        lp = (mlp.forward(th + e, x, nb)[0] * w).sum()
+        lm = (mlp.forward(th - e, x, nb)[0] * w).sum()
+        fd = (lp - lm) / 2e-6
+        assert abs(fd - g[p]) <= 1e-6 * (1 + abs(fd)), (p, fd, g[p])
+
+
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(2)
+    v = np.concatenate([rng.normal(size=2000) * 10.0 ** rng.integers(-6, 6, 2000),
+                        # exact ties between two bf16 neighbours
+                        (np.arange(1, 200) * 2 + 1) * 2.0 ** -8 + 1.0])
+    ref = torch.tensor(v.astype(np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(mlp.bf16(v), ref)
+
+
+def test_features_invariant_under_rotation_about_z():
+    rng = np.random.default_rng(3)
+    N, M = 12, 6
+    neighs = _chain_graph(N, rng)
+    F, G = rng.normal(size=(N, 3)), rng.normal(size=(N, 3))
+    pi = np.zeros((N, 3))
+    pi[:M] = rng.normal(size=(M, 3))
+    pi[:M] /= np.linalg.norm(pi[:M], axis=1, keepdims=True)
+    elem = rng.integers(0, 4, N)
+    h = rng.uniform(2, 5, N)
+    a = 0.7
+    R = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    x1 = mlp.features(elem, F, G, pi, neighs, M, h)
+    x2 = mlp.features(elem, F @ R.T, G @ R.T, pi @ R.T, neighs, M, h)
+    assert np.abs(x1 - x2).max() < 1e-12
+
+
+def test_loss_gradient_matches_finite_differences(orc, nacl_grid):
+    """d/dg of E(x0 + eta*P(g)F0) = -eta F(x').P_g(F0), FP64 force field."""
+    m = native.gen_workload_system(1, 3, 4242, 16, 40, 0.1, nacl_grid)
+    N, M = m["natoms"], m["nnode"]
+    eta = 0.02
+    f0, _, _, E0 = orc.eval(m, m["apos"], grid=nacl_grid)
+    x0, F0 = m["apos"][:N], f0[:N]
+    comp = pc.components(N, m["neighs"])
+    gs = np.full(M, 1.5)
+    gc = np.array([20.0])
+
+    def energy(gs_, gc_):
+        Ft = pc.precondition(x0, F0, m["neighs"], M, gs_, gc_, comp)
+        pos = m["apos"].copy()
+        pos[:N] += eta * Ft
+        f, _, _, E = orc.eval(m, pos, grid=nacl_grid)
+        return E, f[:N]
+
+    E1, F1 = energy(gs, gc)
+    for k in [0, M // 2, M - 1]:
+        S = [k] + [int(j) for j in m["neighs"][k] if j >= 0]
+        ana = -eta * (F1[S] * pc.rigid_project(x0[S], F0[S], 1e-6)).sum()
+        h = 1e-4
+        gp, gm = gs.copy(), gs.copy()
+        gp[k] += h
+        gm[k] -= h
+        fd = (energy(gp, gc)[0] - energy(gm, gc)[0]) / (2 * h)
+        assert abs(fd - ana) <= 1e-6 + 1e-4 * abs(ana), (k, fd, ana)
+    ana = -eta * (F1 * pc.rigid_project(x0, F0, 1e-6)).sum()
+    fd = (energy(gs, gc + 1e-3)[0] - energy(gs, gc - 1e-3)[0]) / 2e-3
+    assert abs(fd - ana) <= 1e-6 + 1e-4 * abs(ana), (fd, ana)
+
+
+# ----------------------------------------------------------------------------- GPU
+def _setup_batch(nsys, seed, nmin, nmax, theta_seed=5):
+    native.set_opt()
+    native.init_workload(1, nsys, seed, nmin, nmax, 0.1, "synth:NaCl", 0.1)
+    th = mlp.init_params(theta_seed, head_scale=1.0)
+    native.set_proposal_weights(th.astype(np.float32))
+    native.set_precond(2, GSTAR, GRIGID)
+    return mlp.bf16(th.astype(np.float32))  # the device's fp32 master, used as bf16 in the dense layers
+
+
+def _systems(B):
+    for s in range(B.nsys):
+        N, M, a0, n0, v0 = (int(v) for v in B.info[s, :5])
+        yield s, N, M, a0, n0, v0
+
+
+@pytest.mark.gpu
+def test_forward_matches_oracle():
+    th = _setup_batch(10, 9100, 16, 200)
+    try:
+        out, feats = native.proposal_forward()
+        B = native.Batch()
+        worst = 0.0
+        for s, N, M, a0, n0, v0 in _systems(B):
+            nb = mlp.neighbour_lists(N, B.neighs[a0:a0 + N])
+            o_ref, _ = mlp.forward(th, feats[a0:a0 + N].astype(np.float64), nb, bf=True)
+            err = np.abs(out[a0:a0 + N] - o_ref).max() / (np.abs(o_ref).max() + 1e-3)
+            worst = max(worst, err)
+        # bf16 activations: an input that rounds to the other neighbour moves an
+        # output by ~2^-8 relative; FP32 accumulation order adds ~1e-6
+        assert worst < 2e-2, worst
+    finally:
+        native.set_precond(0)
+
+
+@pytest.mark.gpu
+def test_features_match_oracle(nacl_grid, orc):
+    _setup_batch(6, 9200, 16, 120)
+    try:
+        _, feats = native.proposal_forward()
+        B = native.Batch()
+        g = native.grid_info()
+        for s, N, M, a0, n0, v0 in _systems(B):
+            pos = B.atoms[v0:v0 + N + M, :3].astype(np.float64)
+            F = B.aforces[v0:v0 + N, :3].astype(np.float64)
+            G = np.zeros((N, 3))
+            for i in range(N):
+                plq = B.PLQs[a0 + i, :3].astype(np.float64)
+                _, G[i] = orc.grid_sample(nacl_grid, pos[i], plq)
+            pi = np.zeros((N, 3))
+            pi[:M] = pos[N:]
+            x = mlp.features(B.elem[a0:a0 + N], F, G, pi, B.neighs[a0:a0 + N], M, pos[:N, 2] - g["pos0"][2])
+            # the device feeds bf16(x) to the matrix cores
+            assert np.abs(feats[a0:a0 + N] - x).max() < 1e-2 * (1 + np.abs(x).max())
+    finally:
+        native.set_precond(0)
+
+
+@pytest.mark.gpu
+def test_backward_matches_oracle():
+    th = _setup_batch(8, 9300, 16, 180)
+    try:
+        out, feats = native.proposal_forward()
+        B = native.Batch()
+        rng = np.random.default_rng(7)
+        dout = rng.normal(size=out.shape).astype(np.float32)
+        g_dev = native.proposal_backward(dout).astype(np.float64)
+        g_ref = np.zeros(mlp.NPARAM)
+        for s, N, M, a0, n0, v0 in _systems(B):
+            nb = mlp.neighbour_lists(N, B.neighs[a0:a0 + N])
+            _, cache = mlp.forward(th, feats[a0:a0 + N].astype(np.float64), nb, bf=True)
+            g_ref += mlp.backward(th, cache, dout[a0:a0 + N].astype(np.float64), nb, bf=True)
+        g_ref /= B.nsys
+        # blocks of the parameter vector: each compared in relative L2 norm
+        o = 0
+        for name, sh in mlp.PARAM_SHAPES:
+            n = int(np.prod(sh))
+            a, b = g_dev[o:o + n], g_ref[o:o + n]
+            rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-12)
+            assert rel < 2e-2, (name, rel)
+            o += n
+    finally:
+        native.set_precond(0)
+
+
+@pytest.mark.gpu
+def test_loss_gradient_matches_oracle(nacl_grid, orc):
+    """Device dloss/doutputs and loss vs the FP64 oracle chain (forces,
+    preconditioner, energy at the displaced geometry)."""
+    _setup_batch(6, 9400, 16, 100)
+    eta, dmax = 0.02, 0.3
+    try:
+        out, feats = native.proposal_forward()
+        dout, loss = native.proposal_loss_grad(eta, dmax)
+        B = native.Batch()
+        for s, N, M, a0, n0, v0 in _systems(B):
+            m = B.system(s)
+            f0, _, _, E0 = orc.eval(m, m["apos"], grid=nacl_grid)
+            x0, F0 = m["apos"][:N], f0[:N]
+            comp = pc.components(N, m["neighs"])
+            o = out[a0:a0 + N].astype(np.float64)
+            gs, gc = mlp.gains(o, M, comp, GSTAR, GRIGID)
+            Ft = pc.precondition(x0, F0, m["neighs"], M, gs, gc, comp)
+            d = eta * Ft
+            dl = np.linalg.norm(d, axis=1)
+            d[dl > dmax] *= (dmax / dl[dl > dmax])[:, None]
+            pos = m["apos"].copy()
+            pos[:N] += d
+            f1, _, _, E1 = orc.eval(m, pos, grid=nacl_grid)
+            F1 = f1[:N]
+            assert abs(loss[s] - (E1 - E0) / N) < 1e-4 + 1e-3 * abs((E1 - E0) / N)
+            dgs = np.array([-eta / N * (F1[S] * pc.rigid_project(x0[S], F0[S], 1e-6)).sum()
+                            for S in ([k] + [int(j) for j in m["neighs"][k] if j >= 0] for k in range(M))])
+            dgc = np.array([-eta / N * (F1[comp == c] * pc.rigid_project(x0[comp == c], F0[comp == c], 1e-6)).sum()
+                            for c in range(int(comp.max()) + 1)])
+            ref = mlp.dgains_do(o, M, comp, GSTAR, GRIGID, dgs, dgc)
+            scale = np.abs(ref).max(0) + 1e-9
+            err = (np.abs(dout[a0:a0 + N] - ref) / scale).max()
+            assert err < 1e-2, (s, err)
+    finally:
+        native.set_precond(0)
+
+
+@pytest.mark.gpu
+def test_learned_relaxation_converges_to_oracle_minima(nacl_grid, orc):
+    """Random network weights (gains stay within their ceilings): every system
+    converges on the exact force, and the energy at the relaxed geometry
+    agrees with the FP64 oracle."""
+    _setup_batch(24, 9500, 16, 120)
+    try:
+        native.set_force_limit(10.0)
+        native.set_opt(0.1, 0.02, 0.2, 1.1, 0.5, 0.8, 5)
+        native.relax(10000, 1e-3, 0, True, True)
+        B = native.Batch()
+        assert (np.array(B.status) == native.ST_CONVERGED).all()
+        assert native.last_chunks() > 0
+        for s in range(0, 24, 4):
+            m = B.system(s)
+            f, _, _, E = orc.eval(m, m["apos"], grid=nacl_grid)
+            N = m["natoms"]
+            assert np.sqrt((f[:N] ** 2).sum(1).max()) < 2e-3
+            assert abs(B.E[s] - E) < 1e-4 * max(1.0, abs(E)) + 1e-4
+    finally:
+        native.set_precond(0)
+        native.set_force_limit(0.0)
+        native.set_opt()
+
+
+@pytest.mark.gpu
+def test_training_lowers_the_loss():
+    _setup_batch(64, 9600, 16, 128, theta_seed=11)
+    try:
+        native.set_opt(0.1, 0.02, 0.2, 1.1, 0.5, 0.8, 5)
+        native.set_force_limit(10.0)
+        native.set_precond(0)
+        native.record_snapshots(50, 40)
+        native.relax(2000, 1e-3, 0, True, False)
+        native.set_precond(2, GSTAR, GRIGID)
+        hist = native.train_proposal(60, 3e-3, 0.02, 0.3, seed=1)
+        assert np.isfinite(hist).all()
+        assert hist[-15:].mean() < hist[:15].mean()
+    finally:
+        native.record_snapshots(1, 0)
+        native.set_precond(0)
+        native.set_force_limit(0.0)
+        native.set_opt()
