@@ -1,4 +1,4 @@
-"""Benchmark: relaxations/sec to Fmax < 1e-3 eV/A on 4096 mixed systems.
+"""Benchmark: relaxations/sec to Fmax < 1e-3 eV/A on 4096 mixed systems, learned vs plain.
 
 Workload (the headline configuration): 4096 independent synthetic molecules,
 N ~ U[16,256] atoms (seeds 10001..14096), each placed 3.0 +- 0.3 A above a
@@ -6,11 +6,18 @@ rigid NaCl(001) slab represented by a 56x56x200 cubic B-spline GridFF
 (Pauli/London/Coulomb, 0.1 A spacing).  FP32 forces, per-system FIRE
 (pyBall/MMFF_multi.py set_opt defaults, per-vec force clamp 10 eV/A as the
 reference GPU update kernel), each system stops at its own max|f| < 1e-3
-eV/A, at most 10000 steps.  Synthetic data.
+eV/A (exact force), at most 10000 steps.  Synthetic data.
 
+Forms, each timed over the same number of passes:
+  plain    -- FIRE on the exact force (the reference dynamics)
+  learned  -- FIRE on the preconditioned force whose gains the proposal network
+              (firecore_amd/proposal_weights.npy, trained on seeds 1,000,001..)
+              re-evaluates inside the relaxation kernel every P steps (P stored
+              with the weights, picked on the training systems)
+  constant -- the same preconditioner with constant gains (ablation)
 One "step" of this benchmark is one full pass of the hot path over the batch:
 every system relaxed from its start geometry to convergence (or the step
-limit) by the persistent kernel, inputs already resident in HBM.
+limit), inputs already resident in HBM.  `value` is the learned form.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu]
 """
@@ -32,10 +39,18 @@ FCONV = 1e-3
 NSTEP_MAX = 10000
 FIRE = (0.1, 0.02, 0.2, 1.1, 0.5, 0.8, 5)  # dt_max, dt_min, damp_max, finc, fdec, falpha, minLastNeg
 FLIMIT = 10.0
+GSTAR_CONST, GRIGID_CONST = 2.0, 63.0
+WEIGHTS = os.path.join(ROOT, "firecore_amd", "proposal_weights.npy")
 PEAK_FP32_TFLOPS = 157.3
 # algorithmic work per unit (FLOP): one unique non-excluded LJ+Coulomb pair,
-# one 64-tap B-spline GridFF sample, one bonded node, one FIRE-updated vec
+# one 64-tap B-spline GridFF sample, one bonded node, one FIRE-updated vec;
+# preconditioner per node (star projection) and per atom (molecule
+# projection); network per atom and evaluation (dense layers 16->64, 128->64
+# x2, head 64->2: 2*(16*64 + 2*128*64 + 2*64))
 FLOP_PAIR, FLOP_GRID, FLOP_NODE, FLOP_VEC = 38, 740, 400, 40
+FLOP_PC_NODE, FLOP_PC_ATOM = 200, 60
+FLOP_MLP_ATOM = 2 * (16 * 64 + 2 * 128 * 64 + 2 * 64)
+PROP_EVERY = 8
 
 
 def log(*a):
@@ -66,8 +81,58 @@ def cpu_baseline(native, grid, nsample, nthreads):
     dt = time.perf_counter() - t
     conv = int(np.sum(np.isfinite(Fm) & (Fm < FCONV)))
     return dict(value=conv / dt, unit="relaxations/s", cores=nthreads, kind="port",
-                sample="%d of the 4096 systems (evenly spaced indices), FP64 oracle, same FIRE/limits; "
-                       "%d converged in %.1f s" % (nsample, conv, dt))
+                sample="%d of the 4096 systems (evenly spaced indices), FP64 oracle of the reference CPU path, "
+                       "plain FIRE, same limits; %d converged in %.1f s" % (nsample, conv, dt))
+
+
+def measure(native, sync, steps, warmup, tag):
+    for w in range(warmup):
+        native.relax(NSTEP_MAX, FCONV, 0, True, False)
+        log("%s warmup %d: %.1f ms" % (tag, w, native.last_ms()))
+    sync()
+    t = time.perf_counter()
+    dev_ms, class_ms = [], []
+    for k in range(steps):
+        native.relax(NSTEP_MAX, FCONV, 0, True, False)
+        dev_ms.append(native.last_ms())
+        class_ms.append([c["ms"] for c in native.class_info()])
+        log("%s step %d: %.1f ms" % (tag, k, dev_ms[-1]))
+    sync()
+    wall = time.perf_counter() - t
+    native.lib().download(True, True)
+    B = native.Batch()
+    status = np.array(B.status)
+    nsteps = np.array(B.nsteps)
+    conv = int((status == native.ST_CONVERGED).sum())
+    ms_per_step = wall * 1e3 / steps
+    return dict(ms_per_step=ms_per_step, dev_ms=float(np.mean(dev_ms)), class_ms=class_ms, classes=native.class_info(),
+                status=status, nsteps=nsteps, conv=conv, value=conv / (ms_per_step / 1e3), B=B)
+
+
+def roofline(r, learned, every=PROP_EVERY):
+    """Achieved FLOP/s of the dominant kernel (the size class with the longest
+    launch) over its measured average launch time."""
+    B, steps, classes = r["B"], r["nsteps"], r["classes"]
+    flops = []
+    for c in classes:
+        f = 0.0
+        for s in c["order"]:
+            N, M, V, P = topo_counts(B, int(s))
+            st = float(steps[s] + 1)
+            f += st * float(FLOP_PAIR * P + FLOP_GRID * N + FLOP_NODE * M + FLOP_VEC * V)
+            if learned:
+                f += st * float(FLOP_PC_NODE * M + FLOP_PC_ATOM * N)
+                f += (st // every + 1) * float(FLOP_MLP_ATOM * N)
+        flops.append(f)
+    dom = int(np.argmax([c["ms"] for c in classes]))
+    dom_ms = float(np.mean([cm[dom] for cm in r["class_ms"]]))
+    achieved = flops[dom] / (dom_ms * 1e-3) / 1e12
+    kind = "2, learned" if learned else "0, plain"
+    return dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=None,
+                kernel="relax_persistent_kernel<%d,HB,%s>" % (classes[dom]["block"], kind),
+                kernel_ms=round(dom_ms, 3), kernel_systems=classes[dom]["nsys"],
+                note="FP32 vector-ALU bound (non-bonded pair loop); peak = MI355X FP32 vector")
 
 
 def main():
@@ -77,6 +142,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=160)
+    ap.add_argument("--forms", default="plain,learned,constant")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
@@ -97,75 +163,61 @@ def main():
     native.init_workload(1, NSYS, SEED0, NMIN, NMAX, 0.1, "synth:NaCl", 0.1)
     log("init %.1f s" % (time.perf_counter() - t0))
     grid = native.grid_info()
+    forms = args.forms.split(",")
+    res = {}
+    if "plain" in forms:
+        native.set_precond(0)
+        res["plain"] = measure(native, sync, args.steps, args.warmup, "plain")
+    meta = None
+    if "learned" in forms and os.path.exists(WEIGHTS):
+        meta = json.load(open(os.path.splitext(WEIGHTS)[0] + ".json"))
+        native.set_proposal_weights(np.load(WEIGHTS))
+        native.set_proposal_every(int(meta.get("every", PROP_EVERY)))
+        native.set_precond(2, meta["gstar"], meta["grigid"])
+        res["learned"] = measure(native, sync, args.steps, args.warmup, "learned")
+    if "constant" in forms:
+        native.set_precond(1, GSTAR_CONST, GRIGID_CONST)
+        res["constant"] = measure(native, sync, args.steps, args.warmup, "constant")
+    native.set_precond(0)
 
-    for w in range(args.warmup):
-        native.relax(NSTEP_MAX, FCONV, 0, True, False)
-        log("warmup %d: %.1f ms" % (w, native.last_ms()))
-
-    sync()
-    t = time.perf_counter()
-    dev_ms = []
-    class_ms = []
-    for k in range(args.steps):
-        native.relax(NSTEP_MAX, FCONV, 0, True, False)
-        dev_ms.append(native.last_ms())
-        class_ms.append([c["ms"] for c in native.class_info()])
-        log("step %d: %.1f ms" % (k, dev_ms[-1]))
-    sync()
-    wall = time.perf_counter() - t
-    ms_per_step = wall * 1e3 / args.steps
-
-    # results of the last pass
-    native.lib().download(True, True)
-    B = native.Batch()
-    status = np.array(B.status)
-    steps = np.array(B.nsteps)
-    conv = int((status == native.ST_CONVERGED).sum())
-    value = conv / (ms_per_step / 1e3)
-
-    # roofline of the dominant kernel (the largest size class of the persistent kernel)
-    classes = native.class_info()
-    flops = []
-    for c in classes:
-        f = 0.0
-        for s in c["order"]:
-            N, M, V, P = topo_counts(B, int(s))
-            f += float(steps[s] + 1) * float(FLOP_PAIR * P + FLOP_GRID * N + FLOP_NODE * M + FLOP_VEC * V)
-        flops.append(f)
-    dom = int(np.argmax([c["ms"] for c in classes]))
-    dom_ms = float(np.mean([cm[dom] for cm in class_ms]))
-    achieved = flops[dom] / (dom_ms * 1e-3) / 1e12
-    roofline = dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=None,
-                    kernel="relax_persistent_kernel<%d>" % classes[dom]["block"],
-                    kernel_ms=round(dom_ms, 3), kernel_systems=classes[dom]["nsys"])
-
+    head = "learned" if "learned" in res else ("plain" if "plain" in res else list(res)[0])
+    r = res[head]
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(native, grid, args.cpu_sample, min(os.cpu_count() or 1, 16))
         log("cpu baseline: %s" % cpu)
 
+    config = {"workload": "4096 uneven molecules N~U[16,256] on NaCl(001) GridFF 56x56x200, FIRE to "
+                          "max|f|<1e-3 eV/A, <=10000 steps",
+              "form": head, "systems": NSYS, "converged": r["conv"],
+              "max_step": int((r["status"] == native.ST_MAXSTEP).sum()),
+              "nonfinite": int((r["status"] == native.ST_NAN).sum()),
+              "mean_steps": round(float(r["nsteps"].mean()), 1),
+              "device_ms_per_step": round(r["dev_ms"], 3)}
+    for k, v in res.items():
+        config["%s_relax_per_s" % k] = round(v["value"], 2)
+        config["%s_mean_steps" % k] = round(float(v["nsteps"].mean()), 1)
+        config["%s_converged" % k] = v["conv"]
+    if "plain" in res and head != "plain":
+        config["%s_over_plain" % head] = round(r["value"] / res["plain"]["value"], 3)
+    if meta:
+        config["proposal"] = "every %d steps; gain ceilings star %g, molecule %g; trained on %d pairs from seeds %s" % (
+            int(meta.get("every", PROP_EVERY)), meta["gstar"], meta["grigid"], meta["pairs"], meta["seeds"])
     out = {
         "metric": "relaxations/sec to Fmax<1e-3 eV/Å, 4096 mixed systems; learned vs plain",
-        "value": round(value, 2),
+        "value": round(r["value"], 2),
         "unit": "relaxations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(r["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "4096 uneven molecules N~U[16,256] on NaCl(001) GridFF 56x56x200, FIRE to "
-                               "max|f|<1e-3 eV/A, <=10000 steps",
-                   "form": "plain", "systems": NSYS, "converged": conv,
-                   "max_step": int((status == native.ST_MAXSTEP).sum()),
-                   "nonfinite": int((status == native.ST_NAN).sum()),
-                   "mean_steps": round(float(steps.mean()), 1),
-                   "device_ms_per_step": round(float(np.mean(dev_ms)), 3)},
-        "roofline": roofline,
+        "config": config,
+        "roofline": roofline(r, head == "learned", int(meta.get("every", PROP_EVERY)) if meta else PROP_EVERY),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -935,6 +935,20 @@ int train_proposal(int iters, double lr, double eta, double dmax, long long seed
     return W.download_theta() ? 0 : -1;
 }
 
+// 2 = evaluate the proposal network inside the persistent relaxation kernel
+// with its second activation buffer in global memory (default; leaves the
+// kernel's LDS footprint, and so its occupancy, unchanged), 1 = the same with
+// that buffer in LDS where the class's LDS budget allows, 0 = chunked
+// relaunches with a separate network kernel between chunks (graph replay).
+// Returns the previous setting.
+int set_learned_inkernel(int on) {
+    World& W = world();
+    const int old = W.learned_inkernel ? (W.h2lds_allowed ? 1 : 2) : 0;
+    W.learned_inkernel = on != 0;
+    W.h2lds_allowed = on != 2;
+    return old;
+}
+
 // Chunk iterations of the last learned relaxation; 0 after other paths.
 int get_last_chunks() { return world().last_chunks; }
 // 1 = replay the learned-relaxation chunk loop from a captured graph

@@ -76,11 +76,14 @@ __device__ __forceinline__ bf16x4 mlp_a_frag(const unsigned short* h, const int4
     const int kk = 4 * (lane >> 4);
     bf16x4 a;
     if (kc < KS) {
-        const unsigned short* q = h + row * MLP_HS + kc * 16 + kk;
-        a[0] = (short)q[0];
-        a[1] = (short)q[1];
-        a[2] = (short)q[2];
-        a[3] = (short)q[3];
+        // 4 consecutive bf16 = one 8-byte load (rows are 8-byte aligned); rows
+        // past n repeat row n-1 (their results are discarded)
+        const int rr = min(row, n - 1);
+        const uint2 u = *reinterpret_cast<const uint2*>(h + rr * MLP_HS + kc * 16 + kk);
+        a[0] = (short)(u.x & 0xFFFFu);
+        a[1] = (short)(u.x >> 16);
+        a[2] = (short)(u.y & 0xFFFFu);
+        a[3] = (short)(u.y >> 16);
     } else {
         const int4 n4 = (row < n) ? nb[row] : make_int4(-1, -1, -1, -1);
         const int ns[4] = {n4.x, n4.y, n4.z, n4.w};
@@ -88,11 +91,11 @@ __device__ __forceinline__ bf16x4 mlp_a_frag(const unsigned short* h, const int4
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             if (ns[t] < 0) continue;
-            const unsigned short* q = h + ns[t] * MLP_HS + (kc - KS) * 16 + kk;
-            s0 += bf2f(q[0]);
-            s1 += bf2f(q[1]);
-            s2 += bf2f(q[2]);
-            s3 += bf2f(q[3]);
+            const uint2 u = *reinterpret_cast<const uint2*>(h + ns[t] * MLP_HS + (kc - KS) * 16 + kk);
+            s0 += __uint_as_float(u.x << 16);
+            s1 += __uint_as_float(u.x & 0xFFFF0000u);
+            s2 += __uint_as_float(u.y << 16);
+            s3 += __uint_as_float(u.y & 0xFFFF0000u);
         }
         a[0] = (short)f2bf(s0);
         a[1] = (short)f2bf(s1);
@@ -215,15 +218,86 @@ __device__ __forceinline__ void mlp_forward(const MlpDev& M, unsigned short* h, 
     __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// Register-lean forward pass for use inside the relaxation kernel: every wave
+// computes one 16x16 output tile at a time (4 accumulator registers and the
+// weight fragments of the current column block), so the network adds little
+// register pressure to the force loop.  Activations alternate between two
+// row buffers of n rows, hA (LDS) and hG (LDS when the workgroup's LDS budget
+// allows, else a per-workgroup slot in global memory that stays in L2):
+// layer 1 hA -> hG, layer 2 hG -> hA, layer 3 hA -> hG, head from hG.
+template <int BLOCK, int KIN>
+__device__ __forceinline__ void mlp_layer_lean(const unsigned short* hin, const int4* nb, const unsigned short* W,
+                                               const float* bias, int n, unsigned short* hout) {
+    constexpr int NW = BLOCK / 64;
+    constexpr int NK = KIN / 16;
+    constexpr int KS = (KIN == 16) ? 1 : 4;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nrb = (n + 15) >> 4;
+    // a wave takes whole row blocks: the A operand of a row block (its own
+    // rows and the neighbour sums) is built once and reused for the four
+    // column blocks; the weight fragments come from L1/L2
+#pragma unroll 1
+    for (int rb = wid; rb < nrb; rb += NW) {
+        bf16x4 a[NK];
+#pragma unroll
+        for (int kc = 0; kc < NK; kc++) a[kc] = mlp_a_frag<KS>(hin, nb, n, rb, kc, lane);
+#pragma unroll 1
+        for (int cb = 0; cb < 4; cb++) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kc = 0; kc < NK; kc++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kc], mlp_b_frag<KIN>(W, cb, kc, lane), acc, 0, 0, 0);
+            const int col = cb * 16 + (lane & 15);
+            const float bc = bias[col];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = rb * 16 + 4 * (lane >> 4) + i;
+                if (row < n) hout[row * MLP_HS + col] = f2bf(silu(acc[i] + bc));
+            }
+        }
+    }
+}
+
+// Forward pass with the features in columns 0..15 of hA; outputs to out[2n].
+// The global writes of a layer are made visible to the whole workgroup by
+// the fence + barrier that follow them.
+template <int BLOCK>
+__device__ __forceinline__ void mlp_forward_lean(const MlpDev& M, unsigned short* hA, unsigned short* hG,
+                                                 const int4* nb, int n, float* out) {
+    mlp_layer_lean<BLOCK, 16>(hA, nb, M.W1, M.b1, n, hG);
+    __threadfence_block();
+    __syncthreads();
+    mlp_layer_lean<BLOCK, 128>(hG, nb, M.W2, M.b2, n, hA);
+    __syncthreads();
+    mlp_layer_lean<BLOCK, 128>(hA, nb, M.W3, M.b3, n, hG);
+    __threadfence_block();
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+        float o0 = M.b4[0], o1 = M.b4[1];
+        const unsigned short* q = hG + i * MLP_HS;
+#pragma unroll 8
+        for (int k = 0; k < MLP_NH; k++) {
+            const float v = bf2f(q[k]);
+            o0 += M.W4[k] * v;
+            o1 += M.W4[MLP_NH + k] * v;
+        }
+        out[i * 2] = o0;
+        out[i * 2 + 1] = o1;
+    }
+    __syncthreads();
+}
+
 // Feature row of atom i (bf16, columns 0..15 of its activation row).  F is
 // the exact atom force, G its substrate part (recomputed here), pi the atom's
 // pi-vector (nodes only), Fb the mean exact force of its bonded neighbours.
 __device__ __forceinline__ void mlp_feature_row(const RunParams& P, const float4* s_pos, const float4* s_F,
                                                 const int4* nb, const int* elem, int N, int M, int i, float4 plq,
-                                                unsigned short* hrow) {
+                                                unsigned short* hrow, const V3* Gpre = nullptr) {
     const V3 p = xyz(s_pos[i]);
     V3 G = zero3();
-    if (P.bGridFF) grid_sample(P.grid, p, mk(plq.x, plq.y, plq.z), G);
+    if (Gpre) G = *Gpre;  // substrate force already sampled at this geometry
+    else if (P.bGridFF) grid_sample(P.grid, p, mk(plq.x, plq.y, plq.z), G);
     const int4 n4 = nb[i];
     const int ns[4] = {n4.x, n4.y, n4.z, n4.w};
     V3 Fb = zero3();
@@ -244,20 +318,29 @@ __device__ __forceinline__ void mlp_feature_row(const RunParams& P, const float4
 }
 
 // Network evaluation inside the relaxation: features from the exact forces in
-// s_F, forward pass, gains into s_gain (per node, then per component at
+// s_F and the substrate force Gi of the calling thread's atom (sampled in the
+// same step), forward pass, gains into s_gain (per node, then per component at
 // s_gain[M + c]).  Uses s_cred (component partial sums) and s_csum[c*16] (the
 // component's atom count) from precondition().  Every thread calls it.
 template <int BLOCK>
-__device__ __forceinline__ void propose_gains(const RunParams& P, const int4* nb, const int* elem, const float4* s_pos,
-                              const float4* s_F, unsigned short* h, float* s_out, float* s_gain, float* s_cred,
-                              const float* s_csum, int N, int M, int nc, int myc, float4 plq) {
+__device__ __forceinline__ void propose_gains(const RunParams& P, const int4* nb, const float4* req, const int* elem,
+                              const float4* s_pos, const float4* s_F, float4* s_req, unsigned short* h,
+                              unsigned short* hG, float* s_out, float* s_gain, float* s_cred, const float* s_csum,
+                              int N, int M, int nc, int myc, float4 plq, V3 Gi) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wid = tid >> 6;
     constexpr int NW = BLOCK / 64;
-    if (tid < N) mlp_feature_row(P, s_pos, s_F, nb, elem, N, M, tid, plq, h + tid * MLP_HS);
+    // the bonded neighbour lists are read for every operand fragment: stage
+    // them in LDS over the (constant) REQ rows, restored from req below
+    int4* s_nb = reinterpret_cast<int4*>(s_req);
+    if (tid < N) {
+        mlp_feature_row(P, s_pos, s_F, nb, elem, N, M, tid, plq, h + tid * MLP_HS, &Gi);
+        s_nb[tid] = nb[tid];
+    }
     __syncthreads();
-    mlp_forward<BLOCK>(P.mlp, h, nb, N, s_out, nullptr, nullptr);
+    mlp_forward_lean<BLOCK>(P.mlp, h, hG, s_nb, N, s_out);
+    if (tid < N) s_req[tid] = req[tid];
     if (tid < M) s_gain[tid] = P.pc_gstar * sigmoidf_(s_out[tid * 2]);
     for (int c = 0; c < nc; c++) {
         const float v = wave_sum((tid < N && myc == c) ? s_out[tid * 2 + 1] : 0.f);

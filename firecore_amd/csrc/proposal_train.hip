@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(256) k_train_fwd(BatchDev B, RunParams P, Trai
     constexpr int BLOCK = 256;
     __shared__ float4 s_pos[2 * BLOCK];
     __shared__ float4 s_F[BLOCK];
-    __shared__ unsigned short s_h[BLOCK * MLP_HS];
+    __shared__ __align__(16) unsigned short s_h[BLOCK * MLP_HS];
     __shared__ float s_out[2 * BLOCK];
     const int s = blockIdx.x;
     const SysInfo I = B.info[s];
@@ -330,8 +330,8 @@ __device__ __forceinline__ void bwd_bias_grad(const unsigned short* dZ, int n, f
 
 __global__ void __launch_bounds__(256) k_train_bwd(BatchDev B, RunParams P, TrainBufs T) {
     constexpr int BLOCK = 256;
-    __shared__ unsigned short s_h[BLOCK * MLP_HS];
-    __shared__ unsigned short s_dz[BLOCK * MLP_HS];
+    __shared__ __align__(16) unsigned short s_h[BLOCK * MLP_HS];
+    __shared__ __align__(16) unsigned short s_dz[BLOCK * MLP_HS];
     __shared__ float s_do[2 * BLOCK];
     const int s = blockIdx.x;
     const SysInfo I = B.info[s];
@@ -485,7 +485,7 @@ __global__ void __launch_bounds__(256) k_mlp_gains(BatchDev B, RunParams P, floa
     constexpr int BLOCK = 256;
     __shared__ float4 s_pos[2 * BLOCK];
     __shared__ float4 s_F[BLOCK];
-    __shared__ unsigned short s_h[BLOCK * MLP_HS];
+    __shared__ __align__(16) unsigned short s_h[BLOCK * MLP_HS];
     __shared__ float s_out[2 * BLOCK];
     const int s = blockIdx.x;
     if (B.status[s] != ST_RUNNING) return;

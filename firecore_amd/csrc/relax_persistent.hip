@@ -21,6 +21,7 @@
 //      MMFFsp3_loc.h:1264-1281)
 #include "kernels.h"
 #include "precond.h"
+#include "proposal.h"
 
 namespace fcd {
 
@@ -37,9 +38,13 @@ struct BlockRed {
 //   fself float4[2*mmax]        bonded self forces of nodes and pi-vectors
 //   fng   float4[8*mmax]        recoils (sigma rows, then pi rows)
 //   mask  uint[wmax*nmax]       rotated exclusion rows, [word][atom]
-//   gain  float[mmax+PC_MAXC]   learned gains of the stars and the components
+//   learned gains only:
+//   gain  float[mmax+PC_MAXC]   gains of the stars and the components
+//   out   float[2*nmax]         network outputs
+//   h     bf16 rows            network activations (one of the two buffers;
+//                              aliased onto facc..fng when it fits there)
 struct LdsLayout {
-    int nmax, mmax, wmax, nwave, learned;
+    int nmax, mmax, wmax, nwave, learned, h2lds;
     __host__ __device__ static int r16(int b) { return (b + 15) & ~15; }
     __host__ __device__ int vmax() const { return nmax + mmax; }
     __host__ __device__ int off_pos() const { return 0; }
@@ -51,7 +56,18 @@ struct LdsLayout {
     __host__ __device__ int off_mask() const { return off_fng() + 8 * mmax * 16; }
     __host__ __device__ int end_base() const { return off_mask() + nmax * wmax * 4; }
     __host__ __device__ int off_gain() const { return r16(end_base()); }
-    __host__ __device__ int bytes() const { return learned ? off_gain() + r16((mmax + PC_MAXC) * 4) : end_base(); }
+    __host__ __device__ int off_out() const { return off_gain() + r16((mmax + PC_MAXC) * 4); }
+    __host__ __device__ int off_hx() const { return off_out() + r16(nmax * 8); }
+    __host__ __device__ int hbytes() const { return r16(nmax * MLP_HS * 2); }
+    // first activation buffer: in LDS when it fits over facc..mask (free during
+    // the network evaluation; the exclusion rows are staged again after it),
+    // else in the workgroup's global slot
+    __host__ __device__ bool h_aliased() const { return hbytes() <= end_base() - off_facc(); }
+    __host__ __device__ int off_h2() const { return off_hx(); }
+    __host__ __device__ int bytes() const {
+        if (!learned) return end_base();
+        return off_h2() + ((h2lds && h_aliased()) ? hbytes() : 0);
+    }
 };
 
 // Work queues: class c owns order[qoff[c] .. qoff[c]+qcnt[c]) and counter
@@ -64,10 +80,11 @@ struct Queues {
     int qcnt[4];
     int* counter;
     int lowest;  // smallest class index a workgroup may steal from (its own index: no stealing)
+    unsigned short* hslot;  // learned gains: global activation rows, one slot per workgroup of this launch
 };
 
-template <int BLOCK, bool HB, bool PC>
-__global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, RunParams P, Queues Q, int cls,
+template <int BLOCK, bool HB, int PC>
+__global__ void __launch_bounds__(BLOCK, PC ? 2 : 3) relax_persistent_kernel(BatchDev B, RunParams P, Queues Q, int cls,
                                                                  LdsLayout L) {
     extern __shared__ float4 lds_dyn[];
     char* lds = reinterpret_cast<char*>(lds_dyn);
@@ -79,7 +96,15 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
     unsigned* s_mask = reinterpret_cast<unsigned*>(lds + L.off_mask());
     float4* s_facc = reinterpret_cast<float4*>(lds + L.off_facc());
     float* s_gain = reinterpret_cast<float*>(lds + L.off_gain());
-    const bool learned = PC && L.learned && P.pc_mode == 2;
+    float* s_out = reinterpret_cast<float*>(lds + L.off_out());
+    // activation buffers of the in-kernel network: LDS where the layout allows,
+    // else the two halves of this workgroup's global slot
+    unsigned short* g_slot = Q.hslot ? Q.hslot + (size_t)blockIdx.x * L.hbytes() : nullptr;
+    unsigned short* s_h = L.h_aliased() ? reinterpret_cast<unsigned short*>(lds + L.off_facc()) : g_slot;
+    unsigned short* g_h = (L.h2lds && L.h_aliased()) ? reinterpret_cast<unsigned short*>(lds + L.off_h2())
+                                                     : (g_slot ? g_slot + L.hbytes() / 2 : nullptr);
+    const bool learned = PC == 2 && L.learned && P.pc_mode == 2;
+    const bool inkernel = learned && P.chunk == 0;  // network evaluated inside this kernel
     constexpr int NW = BLOCK / 64;
     __shared__ float s_red[4][NW];
     __shared__ double s_redd[NW];
@@ -160,6 +185,7 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
         fs.damping = P.fire.damp_max;
         fs.lastNeg = 0;
         int step = 0;
+        V3 Gk = zero3();  // substrate force of this thread's atom (learned gains only)
         if (P.chunk > 0) {  // resume the suspended state of this system
             const float4 fst = B.fire[s];
             fs.dt = fst.x;
@@ -172,7 +198,7 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
                 if (v < V) vel[r] = xyz(B.avel[I.v0 + v]);
             }
         }
-        if (learned) {
+        if (learned && !inkernel) {  // chunked: gains from the network kernel between chunks
             for (int k = tid; k < M; k += BLOCK) s_gain[k] = B.gnode[I.n0 + k];
             if (tid < PC_MAXC) s_gain[M + tid] = B.gcomp[s * PC_MAXC + tid];
         }
@@ -265,7 +291,15 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
                         f -= d * (fr * pc.w);
                     }
                 }
-                if (P.bGridFF) grid_sample(P.grid, pi, mk(plq.x, plq.y, plq.z), f);
+                if (P.bGridFF) {
+                    if (PC == 2) {  // the network's features reuse the substrate force of this step
+                        Gk = zero3();
+                        grid_sample(P.grid, pi, mk(plq.x, plq.y, plq.z), Gk);
+                        f += Gk;
+                    } else {
+                        grid_sample(P.grid, pi, mk(plq.x, plq.y, plq.z), f);
+                    }
+                }
                 s_fnb[i] = make_float4(f.x, f.y, f.z, 0.f);
             }
             if (hasNode) {
@@ -314,7 +348,18 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
             }
             if (PC)
                 precondition<BLOCK>(P, s_pos, s_fnb, s_fself, s_fng, &s_cred[0][0][0], &s_csum[0][0], s_gain, learned, N,
-                                    M, nc, myc, hasAtom, hasNode, ng, bk[0], fv[0], [] {});
+                                    M, nc, myc, hasAtom, hasNode, ng, bk[0], fv[0], [&] {
+                                        if (PC == 2 && inkernel && step % P.pc_every == 0)
+                                            propose_gains<BLOCK>(P, B.neighs + I.a0, B.REQs + I.a0, B.elem + I.a0,
+                                                                 s_pos, s_fnb, s_req, s_h, g_h, s_out, s_gain,
+                                                                 &s_cred[0][0][0], &s_csum[0][0], N, M, nc, myc, plq,
+                                                                 Gk);
+                                        if (PC == 2 && inkernel && step % P.pc_every == 0 && L.h_aliased()) {
+                                            for (int w = tid; w < N * W; w += BLOCK)
+                                                s_mask[(w % W) * N + (w / W)] = B.excl_rot[I.mask0 + w];
+                                            __syncthreads();
+                                        }
+                                    });
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const int v = tid + r * BLOCK;
@@ -458,11 +503,13 @@ __global__ void __launch_bounds__(BLOCK) relax_persistent_kernel(BatchDev B, Run
 
 #define FC_INST(BL, HB, PCF) \
     template __global__ void relax_persistent_kernel<BL, HB, PCF>(BatchDev, RunParams, Queues, int, LdsLayout);
-#define FC_INST2(BL)          \
-    FC_INST(BL, false, false) \
-    FC_INST(BL, true, false)  \
-    FC_INST(BL, false, true)  \
-    FC_INST(BL, true, true)
+#define FC_INST2(BL)      \
+    FC_INST(BL, false, 0) \
+    FC_INST(BL, true, 0)  \
+    FC_INST(BL, false, 1) \
+    FC_INST(BL, true, 1)  \
+    FC_INST(BL, false, 2) \
+    FC_INST(BL, true, 2)
 FC_INST2(64)
 FC_INST2(128)
 FC_INST2(192)
@@ -470,7 +517,7 @@ FC_INST2(256)
 #undef FC_INST2
 #undef FC_INST
 
-template <bool HB, bool PCF>
+template <bool HB, int PCF>
 static const void* kernel_ptr(int block) {
     switch (block) {
         case 64: return (const void*)&relax_persistent_kernel<64, HB, PCF>;
@@ -480,9 +527,13 @@ static const void* kernel_ptr(int block) {
         default: return nullptr;
     }
 }
+// pc: preconditioner mode 0 (off), 1 (constant gains), 2 (learned gains)
 static const void* kernel_ptr(int block, int hb, int pc) {
-    if (pc) return hb ? kernel_ptr<true, true>(block) : kernel_ptr<false, true>(block);
-    return hb ? kernel_ptr<true, false>(block) : kernel_ptr<false, false>(block);
+    switch (pc) {
+        case 1: return hb ? kernel_ptr<true, 1>(block) : kernel_ptr<false, 1>(block);
+        case 2: return hb ? kernel_ptr<true, 2>(block) : kernel_ptr<false, 2>(block);
+        default: return hb ? kernel_ptr<true, 0>(block) : kernel_ptr<false, 0>(block);
+    }
 }
 
 }  // namespace fcd
@@ -491,8 +542,8 @@ static const void* kernel_ptr(int block, int hb, int pc) {
 extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups, size_t lds_bytes, hipStream_t stream,
                                                  const fcd::BatchDev* B, const fcd::RunParams* P, const int* order,
                                                  const int* qinfo, int* counters, int cls, int nmax, int mmax,
-                                                 int wmax, int steal) {
-    fcd::LdsLayout L{nmax, mmax, wmax, block / 64, P->pc_mode == 2 ? 1 : 0};
+                                                 int wmax, int steal, unsigned short* hslot, int h2lds) {
+    fcd::LdsLayout L{nmax, mmax, wmax, block / 64, P->pc_mode == 2 ? 1 : 0, h2lds};
     fcd::Queues Q;
     Q.order = order;
     for (int c = 0; c < 4; c++) {
@@ -501,15 +552,22 @@ extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups,
     }
     Q.counter = counters;
     Q.lowest = steal ? 0 : cls;
-    const void* k = fcd::kernel_ptr(block, hb, P->pc_mode != 0);
+    Q.hslot = hslot;
+    const void* k = fcd::kernel_ptr(block, hb, P->pc_mode);
     if (!k) return hipErrorInvalidValue;
     void* args[] = {(void*)B, (void*)P, (void*)&Q, (void*)&cls, (void*)&L};
     return hipLaunchKernel(k, dim3(ngroups), dim3(block), args, lds_bytes, stream);
 }
 
-extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block, int learned) {
-    fcd::LdsLayout L{nmax, mmax, wmax, block / 64, learned};
+extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block, int learned, int h2lds) {
+    fcd::LdsLayout L{nmax, mmax, wmax, block / 64, learned, h2lds};
     return L.bytes();
+}
+
+// bf16 elements of one workgroup's global activation slot (two buffers)
+extern "C" int fc_hslot_elems(int nmax) {
+    fcd::LdsLayout L{nmax, 1, 1, 1, 1, 0};
+    return L.hbytes();
 }
 
 extern "C" hipError_t fc_persistent_occupancy(int block, int hb, int pc, size_t lds, int* nblocks) {

@@ -52,7 +52,7 @@ extern "C" hipError_t fc_launch_zero_f4(hipStream_t st, float4* a, int n);
 extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups, size_t lds_bytes, hipStream_t stream,
                                                  const fcd::BatchDev* B, const fcd::RunParams* P, const int* order,
                                                  const int* qinfo, int* counters, int cls, int nmax, int mmax,
-                                                 int wmax, int steal);
+                                                 int wmax, int steal, unsigned short* hslot, int h2lds);
 
 namespace fcw {
 
@@ -225,7 +225,7 @@ int World::relax_learned(int nstep_max, double Fconv, bool restore_start) {
             hipStreamWaitEvent(cs, evs[0], 0);
             if (!hip_ok(fc_launch_relax_persistent(C.block, hb_batch ? 1 : 0, C.ngroups, C.lds, cs, &B, &P, d_order.p,
                                                    qinfo.data(), d_queue.p, C.cls, C.nmax, C.mmax, C.wmax,
-                                                   sched_mode == 0 ? 1 : 0),
+                                                   sched_mode == 0 ? 1 : 0, nullptr, 0),
                         "relax chunk"))
                 return false;
             hipEventRecord(evs[k + 1], cs);

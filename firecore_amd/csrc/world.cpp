@@ -17,8 +17,9 @@
 extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups, size_t lds_bytes, hipStream_t stream,
                                                  const fcd::BatchDev* B, const fcd::RunParams* P, const int* order,
                                                  const int* qinfo, int* counters, int cls, int nmax, int mmax,
-                                                 int wmax, int steal);
-extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block, int learned);
+                                                 int wmax, int steal, unsigned short* hslot, int h2lds);
+extern "C" int fc_hslot_elems(int nmax);
+extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block, int learned, int h2lds);
 extern "C" hipError_t fc_launch_split_step(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P,
                                            const int* node_sys, int ntot_nodes, int maxN, int do_move, int step);
 extern "C" hipError_t fc_launch_reset(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P);
@@ -492,8 +493,20 @@ void World::plan_classes() {
         C.nmax = nrun;
         C.mmax = mrun;
         C.wmax = (C.nmax + 31) / 32;
+        // learned gains: the second activation buffer goes to LDS when the class
+        // keeps two resident workgroups per CU pair of SIMDs (see plan below)
+        C.h2lds = 0;
         C.lds = (size_t)fc_lds_bytes(std::max(C.nmax, 1), std::max(C.mmax, 1), std::max(C.wmax, 1), C.block,
-                                     pc_mode == 2 ? 1 : 0);
+                                     pc_mode == 2 ? 1 : 0, 0);
+        if (pc_mode == 2 && h2lds_allowed) {
+            const size_t budget = (size_t)160 * 1024 / std::max(1, 8 / (C.block / 64));
+            const size_t with = (size_t)fc_lds_bytes(std::max(C.nmax, 1), std::max(C.mmax, 1), std::max(C.wmax, 1),
+                                                     C.block, 1, 1);
+            if (with <= budget) {
+                C.h2lds = 1;
+                C.lds = with;
+            }
+        }
         auto cost = [&](int s) {
             const double N = info[s].natoms;
             const double st = (nsteps.size() == (size_t)nSys && nsteps[s] > 0) ? nsteps[s] + 1.0 : 1000.0 + 20.0 * N;
@@ -517,7 +530,7 @@ void World::plan_classes() {
         C.cls = c;
         if (C.order.empty()) continue;
         int nb = 1;
-        fc_persistent_occupancy(C.block, hb_batch ? 1 : 0, pc_mode != 0 ? 1 : 0, C.lds, &nb);
+        fc_persistent_occupancy(C.block, hb_batch ? 1 : 0, pc_mode, C.lds, &nb);
         nb = std::max(nb, 1);
         C.occ = nb;
         int g = (int)std::lround((double)ncu * nb * C.work / std::max(wtot, 1e-30));
@@ -539,7 +552,11 @@ void World::plan_classes() {
 int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
     last_chunks = 0;
     if (!dev_ready || nSys == 0) return -1;
-    if (pc_mode == 2) return relax_learned(nstep_max, Fconv, restore_start);
+    if (pc_mode == 2 && !learned_inkernel) return relax_learned(nstep_max, Fconv, restore_start);
+    if (pc_mode == 2 && (!d_mlpW.p || maxN > 256 || maxM > 256)) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] learned gains need proposal weights and systems of <= 256 atoms\n");
+        return -1;
+    }
     if (!ring_ok) {
         std::fprintf(stderr, "[libMMFFmulti_lib] asymmetric exclusions: relaxing with the split kernels\n");
         return relax_split(nstep_max, Fconv, restore_start);
@@ -552,6 +569,16 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
     fcd::RunParams P = run_params(Fconv, nstep_max);
     std::vector<int> zeros(8, 0);
     hip_ok(hipMemcpyAsync(d_queue.p, zeros.data(), sizeof(int) * 8, hipMemcpyHostToDevice, stream), "queue reset");
+    // learned gains evaluated in-kernel: one global activation slot per workgroup
+    std::vector<size_t> slot_off(classes.size(), 0);
+    if (pc_mode == 2) {
+        size_t tot = 0;
+        for (size_t k = 0; k < classes.size(); k++) {
+            slot_off[k] = tot;
+            tot += (size_t)classes[k].ngroups * fc_hslot_elems(std::max(classes[k].nmax, 1));
+        }
+        d_hslot.alloc(std::max<size_t>(tot, 1));
+    }
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
@@ -567,7 +594,8 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
         hipEventRecord(es, st);
         hipError_t err = fc_launch_relax_persistent(C.block, hb_batch ? 1 : 0, C.ngroups, C.lds, st, &B, &P, d_order.p,
                                                     qinfo.data(), d_queue.p, C.cls, C.nmax, C.mmax, C.wmax,
-                                                    sched_mode == 0 ? 1 : 0);
+                                                    sched_mode == 0 ? 1 : 0,
+                                                    pc_mode == 2 ? d_hslot.p + slot_off[k] : nullptr, C.h2lds);
         if (!hip_ok(err, "relax_persistent launch")) return -1;
         hipEventRecord(ev, st);
         hipStreamWaitEvent(stream, ev, 0);

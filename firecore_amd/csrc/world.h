@@ -33,6 +33,7 @@ struct SizeClass {
     double work = 0;         // sum of N^2
     int ngroups = 0;
     size_t lds = 0;
+    int h2lds = 0;      // learned gains: second activation buffer in LDS (else global)
     float last_ms = 0;  // device time of this class's launch in the last relax
 };
 
@@ -79,6 +80,9 @@ struct World {
     int relax_learned(int nstep_max, double Fconv, bool restore_start);
     DevArr<int> d_count;
     bool use_graphs = true;
+    bool learned_inkernel = true;  // network inside the persistent kernel (else chunked relaunches)
+    bool h2lds_allowed = false;    // second activation buffer of the in-kernel network may use LDS
+    DevArr<unsigned short> d_hslot;
     int last_chunks = 0;
 
     // ---- systems (host, double) and the template system (system 0)

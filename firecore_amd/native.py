@@ -447,3 +447,10 @@ def set_graphs(on):
     L.set_graphs.argtypes = [c_int]
     L.set_graphs.restype = c_int
     return L.set_graphs(1 if on else 0)
+
+
+def set_learned_inkernel(on):
+    L = lib()
+    L.set_learned_inkernel.argtypes = [c_int]
+    L.set_learned_inkernel.restype = c_int
+    return L.set_learned_inkernel(int(on))

@@ -223,6 +223,9 @@ int proposal_loss_grad(double eta, double dmax, float* dout, float* loss);
 int proposal_backward(const float* dout, float* grad);
 /* Adam training on the recorded snapshots, one snapshot per system per step */
 int train_proposal(int iters, double lr, double eta, double dmax, long long seed, float* loss_hist);
+/* 2 = proposal network inside the persistent kernel, activations spill to global (default);
+ * 1 = the same with LDS activations where the LDS budget allows; 0 = chunked relaunches */
+int set_learned_inkernel(int on);
 /* chunk iterations of the last learned relaxation */
 int get_last_chunks(void);
 /* 1 = replay the learned chunk loop from a captured HIP graph, 0 = direct launches */
