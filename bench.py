@@ -42,6 +42,8 @@ FLIMIT = 10.0
 GSTAR_CONST, GRIGID_CONST = 2.0, 63.0
 WEIGHTS = os.path.join(ROOT, "firecore_amd", "proposal_weights.npy")
 PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+GRID_BYTES_ATOM = 64 * 16  # one B-spline GridFF sample: 4x4x4 float4 coefficients
 # algorithmic work per unit (FLOP): one unique non-excluded LJ+Coulomb pair,
 # one 64-tap B-spline GridFF sample, one bonded node, one FIRE-updated vec;
 # preconditioner per node (star projection) and per atom (molecule
@@ -110,29 +112,48 @@ def measure(native, sync, steps, warmup, tag):
 
 
 def roofline(r, learned, every=PROP_EVERY):
-    """Achieved FLOP/s of the dominant kernel (the size class with the longest
-    launch) over its measured average launch time."""
+    """Roofline of the dominant kernel (the size class with the longest launch).
+
+    The persistent kernel keeps every system's state in LDS/registers; its
+    memory traffic is the B-spline GridFF gathers, 64 float4 coefficients
+    (1024 B) per atom and step from a 10 MB grid that does not stay in the
+    per-XCD L2.  achieved = those algorithmic gather bytes of the launch over
+    its measured average duration (HIP events on the launch's stream), against
+    the HBM peak; the FP32 work of the same launch is reported beside it.
+    traffic = L2-to-fabric bytes per launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated; profiles/), when recorded."""
     B, steps, classes = r["B"], r["nsteps"], r["classes"]
-    flops = []
+    flops, gbytes = [], []
     for c in classes:
         f = 0.0
+        g = 0.0
         for s in c["order"]:
             N, M, V, P = topo_counts(B, int(s))
             st = float(steps[s] + 1)
             f += st * float(FLOP_PAIR * P + FLOP_GRID * N + FLOP_NODE * M + FLOP_VEC * V)
+            g += st * float(GRID_BYTES_ATOM * N)
             if learned:
                 f += st * float(FLOP_PC_NODE * M + FLOP_PC_ATOM * N)
                 f += (st // every + 1) * float(FLOP_MLP_ATOM * N)
         flops.append(f)
+        gbytes.append(g)
     dom = int(np.argmax([c["ms"] for c in classes]))
     dom_ms = float(np.mean([cm[dom] for cm in r["class_ms"]]))
-    achieved = flops[dom] / (dom_ms * 1e-3) / 1e12
-    kind = "2, learned" if learned else "0, plain"
-    return dict(bound="mfma", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=None,
-                kernel="relax_persistent_kernel<%d,HB,%s>" % (classes[dom]["block"], kind),
+    achieved = gbytes[dom] / (dom_ms * 1e-3) / 1e9
+    tflops = flops[dom] / (dom_ms * 1e-3) / 1e12
+    kind = 2 if learned else 0
+    name = "relax_persistent_kernel<%d,false,%d>" % (classes[dom]["block"], kind)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "relax_persistent_traffic.json")
+    if os.path.exists(tf):
+        rec = json.load(open(tf)).get(name)
+        if rec:
+            traffic = rec["bytes_per_launch"]
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=round(achieved / PEAK_HBM_GBS, 4), traffic=traffic, kernel=name,
                 kernel_ms=round(dom_ms, 3), kernel_systems=classes[dom]["nsys"],
-                note="FP32 vector-ALU bound (non-bonded pair loop); peak = MI355X FP32 vector")
+                algorithmic_bytes=gbytes[dom], fp32_tflops=round(tflops, 3),
+                fp32_frac=round(tflops / PEAK_FP32_TFLOPS, 4))
 
 
 def main():
@@ -143,6 +164,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=160)
     ap.add_argument("--forms", default="plain,learned,constant")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="copy a 256 MiB buffer once first (the known-size kernel that scales memory counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
@@ -156,6 +179,13 @@ def main():
     except Exception:  # torch is plumbing only
         sync = lambda: None  # noqa: E731
 
+    if args.calibrate_copy:
+        import torch
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     native.set_opt(*FIRE)
     native.set_force_limit(FLIMIT)
     native.set_conv_mode(0)
