@@ -142,7 +142,7 @@ def roofline(r, learned, every=PROP_EVERY):
     achieved = gbytes[dom] / (dom_ms * 1e-3) / 1e9
     tflops = flops[dom] / (dom_ms * 1e-3) / 1e12
     kind = 2 if learned else 0
-    name = "relax_persistent_kernel<%d,false,%d>" % (classes[dom]["block"], kind)
+    name = "relax_persistent_kernel<%d, false, %d>" % (classes[dom]["block"], kind)
     traffic = None
     tf = os.path.join(ROOT, "profiles", "relax_persistent_traffic.json")
     if os.path.exists(tf):

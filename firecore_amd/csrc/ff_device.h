@@ -292,6 +292,102 @@ __device__ __forceinline__ float grid_sample(const GridDev& g, V3 p, V3 plq, V3&
     return E;
 }
 
+// The same sample with the atom's 4x4x4 window of channel-combined
+// coefficients (cs = plq . c, 64 floats = 256 B) kept in a per-atom cache
+// slot in global memory.  While the atom stays in the same grid cell (the
+// window base `key` is unchanged) the 64 gathers from the 10 MB grid become
+// one contiguous 256 B read that stays in L2; on a cell change the window is
+// gathered, combined and written back.  Same arithmetic as grid_sample.
+// Cubic B-spline basis value and derivative of index a (0..3) at u: the
+// slab loop below runs rolled, so the four weights are computed per slab.
+__device__ __forceinline__ void bs_basis_a(float u, int a, float& b, float& d) {
+    const float inv6 = 1.f / 6.f;
+    const float u2 = u * u;
+    const float t = 1.f - u;
+    switch (a) {
+        case 0: b = inv6 * t * t * t; d = -0.5f * t * t; break;
+        case 1: b = inv6 * (3.f * u2 * (u - 2.f) + 4.f); d = 0.5f * (3.f * u2 - 4.f * u); break;
+        case 2: b = inv6 * (3.f * u * (1.f + u - u2) + 1.f); d = 0.5f * (-3.f * u2 + 2.f * u + 1.f); break;
+        default: b = inv6 * u2 * u; d = 0.5f * u2; break;
+    }
+}
+
+__device__ __forceinline__ float grid_sample_cached(const GridDev& g, V3 p, V3 plq, float4* cache, int& key, V3& f) {
+    const float ux = (p.x - g.p0x) * g.isx;
+    const float uy = (p.y - g.p0y) * g.isy;
+    const float uz = (p.z - g.p0z) * g.isz;
+    int ix = (int)ux, iy = (int)uy, iz = (int)uz;
+    if (ux < 0.f) ix--;
+    if (uy < 0.f) iy--;
+    const float tx = ux - ix, ty = uy - iy, tz = uz - iz;
+    iz = min(max(iz, 3), g.nz - 5);
+    ix = imod(ix - 1, g.nx);
+    iy = imod(iy - 1, g.ny);
+    const int k = (ix * g.ny + iy) * g.nz + iz;
+    const bool hit = (k == key);
+    key = k;
+    float by[4], dy[4], bz[4], dz[4];
+    bs_basis(ty, by);
+    bs_dbasis(ty, dy);
+    bs_basis(tz, bz);
+    bs_dbasis(tz, dz);
+    float Fx = 0.f, Fy = 0.f, Fz = 0.f, E = 0.f;
+#pragma unroll 1
+    for (int a = 0; a < 4; a++) {  // x-slabs one at a time: 16 coefficients live
+        float cs[16];
+        if (hit) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float4 v = cache[a * 4 + q];
+                cs[q * 4 + 0] = v.x;
+                cs[q * 4 + 1] = v.y;
+                cs[q * 4 + 2] = v.z;
+                cs[q * 4 + 3] = v.w;
+            }
+        } else {
+            int xa = ix + a;
+            if (xa >= g.nx) xa -= g.nx;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                int yb = iy + b;
+                if (yb >= g.ny) yb -= g.ny;
+                const float4* col = g.coefs + ((size_t)(xa * g.ny + yb) * g.nz + (iz - 1));
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const float4 q = col[c];
+                    cs[b * 4 + c] = plq.x * q.x + plq.y * q.y + plq.z * q.z;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                cache[a * 4 + q] = make_float4(cs[q * 4], cs[q * 4 + 1], cs[q * 4 + 2], cs[q * 4 + 3]);
+        }
+        float ey = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            float e = 0.f, de = 0.f;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                e += bz[c] * cs[b * 4 + c];
+                de += dz[c] * cs[b * 4 + c];
+            }
+            fy += e * dy[b];
+            fz += de * by[b];
+            ey += e * by[b];
+        }
+        float bxa, dxa;
+        bs_basis_a(tx, a, bxa, dxa);
+        Fx += ey * dxa;
+        Fy += fy * bxa;
+        Fz += fz * bxa;
+        E += ey * bxa;
+    }
+    f.x -= Fx * g.isx;
+    f.y -= Fy * g.isy;
+    f.z -= Fz * g.isz;
+    return E;
+}
+
 // -------------------------------------------------------------------------
 // FIRE parameters (DynamicOpt.h:47-95 defaults; set_opt, MMFFmulti_lib.cpp:141)
 struct FireParams {

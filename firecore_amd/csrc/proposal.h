@@ -259,12 +259,13 @@ __device__ __forceinline__ void mlp_layer_lean(const unsigned short* hin, const 
     }
 }
 
-// Forward pass with the features in columns 0..15 of hA; outputs to out[2n].
+// Forward pass with the features in columns 0..15 of hA (n <= BLOCK rows);
+// returns the two head outputs of row threadIdx.x (zeros for threads >= n).
 // The global writes of a layer are made visible to the whole workgroup by
 // the fence + barrier that follow them.
 template <int BLOCK>
-__device__ __forceinline__ void mlp_forward_lean(const MlpDev& M, unsigned short* hA, unsigned short* hG,
-                                                 const int4* nb, int n, float* out) {
+__device__ __forceinline__ float2 mlp_forward_lean(const MlpDev& M, unsigned short* hA, unsigned short* hG,
+                                                   const int4* nb, int n) {
     mlp_layer_lean<BLOCK, 16>(hA, nb, M.W1, M.b1, n, hG);
     __threadfence_block();
     __syncthreads();
@@ -273,19 +274,20 @@ __device__ __forceinline__ void mlp_forward_lean(const MlpDev& M, unsigned short
     mlp_layer_lean<BLOCK, 128>(hA, nb, M.W3, M.b3, n, hG);
     __threadfence_block();
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += BLOCK) {
-        float o0 = M.b4[0], o1 = M.b4[1];
+    float2 o = make_float2(0.f, 0.f);
+    const int i = threadIdx.x;
+    if (i < n) {
+        o = make_float2(M.b4[0], M.b4[1]);
         const unsigned short* q = hG + i * MLP_HS;
 #pragma unroll 8
         for (int k = 0; k < MLP_NH; k++) {
             const float v = bf2f(q[k]);
-            o0 += M.W4[k] * v;
-            o1 += M.W4[MLP_NH + k] * v;
+            o.x += M.W4[k] * v;
+            o.y += M.W4[MLP_NH + k] * v;
         }
-        out[i * 2] = o0;
-        out[i * 2 + 1] = o1;
     }
     __syncthreads();
+    return o;
 }
 
 // Feature row of atom i (bf16, columns 0..15 of its activation row).  F is
@@ -325,7 +327,7 @@ __device__ __forceinline__ void mlp_feature_row(const RunParams& P, const float4
 template <int BLOCK>
 __device__ __forceinline__ void propose_gains(const RunParams& P, const int4* nb, const float4* req, const int* elem,
                               const float4* s_pos, const float4* s_F, float4* s_req, unsigned short* h,
-                              unsigned short* hG, float* s_out, float* s_gain, float* s_cred, const float* s_csum,
+                              unsigned short* hG, float* s_gain, float* s_cred, const float* s_csum,
                               int N, int M, int nc, int myc, float4 plq, V3 Gi) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -339,11 +341,11 @@ __device__ __forceinline__ void propose_gains(const RunParams& P, const int4* nb
         s_nb[tid] = nb[tid];
     }
     __syncthreads();
-    mlp_forward_lean<BLOCK>(P.mlp, h, hG, s_nb, N, s_out);
+    const float2 o = mlp_forward_lean<BLOCK>(P.mlp, h, hG, s_nb, N);
     if (tid < N) s_req[tid] = req[tid];
-    if (tid < M) s_gain[tid] = P.pc_gstar * sigmoidf_(s_out[tid * 2]);
+    if (tid < M) s_gain[tid] = P.pc_gstar * sigmoidf_(o.x);
     for (int c = 0; c < nc; c++) {
-        const float v = wave_sum((tid < N && myc == c) ? s_out[tid * 2 + 1] : 0.f);
+        const float v = wave_sum((tid < N && myc == c) ? o.y : 0.f);
         if (lane == 0) s_cred[(wid * PC_MAXC + c) * PC_NSUM] = v;
     }
     __syncthreads();

@@ -56,8 +56,7 @@ struct LdsLayout {
     __host__ __device__ int off_mask() const { return off_fng() + 8 * mmax * 16; }
     __host__ __device__ int end_base() const { return off_mask() + nmax * wmax * 4; }
     __host__ __device__ int off_gain() const { return r16(end_base()); }
-    __host__ __device__ int off_out() const { return off_gain() + r16((mmax + PC_MAXC) * 4); }
-    __host__ __device__ int off_hx() const { return off_out() + r16(nmax * 8); }
+    __host__ __device__ int off_hx() const { return off_gain() + r16((mmax + PC_MAXC) * 4); }
     __host__ __device__ int hbytes() const { return r16(nmax * MLP_HS * 2); }
     // first activation buffer: in LDS when it fits over facc..mask (free during
     // the network evaluation; the exclusion rows are staged again after it),
@@ -84,7 +83,7 @@ struct Queues {
 };
 
 template <int BLOCK, bool HB, int PC>
-__global__ void __launch_bounds__(BLOCK, PC ? 2 : 3) relax_persistent_kernel(BatchDev B, RunParams P, Queues Q, int cls,
+__global__ void __launch_bounds__(BLOCK, PC >= 2 ? 2 : 3) relax_persistent_kernel(BatchDev B, RunParams P, Queues Q, int cls,
                                                                  LdsLayout L) {
     extern __shared__ float4 lds_dyn[];
     char* lds = reinterpret_cast<char*>(lds_dyn);
@@ -96,14 +95,13 @@ __global__ void __launch_bounds__(BLOCK, PC ? 2 : 3) relax_persistent_kernel(Bat
     unsigned* s_mask = reinterpret_cast<unsigned*>(lds + L.off_mask());
     float4* s_facc = reinterpret_cast<float4*>(lds + L.off_facc());
     float* s_gain = reinterpret_cast<float*>(lds + L.off_gain());
-    float* s_out = reinterpret_cast<float*>(lds + L.off_out());
     // activation buffers of the in-kernel network: LDS where the layout allows,
     // else the two halves of this workgroup's global slot
     unsigned short* g_slot = Q.hslot ? Q.hslot + (size_t)blockIdx.x * L.hbytes() : nullptr;
     unsigned short* s_h = L.h_aliased() ? reinterpret_cast<unsigned short*>(lds + L.off_facc()) : g_slot;
     unsigned short* g_h = (L.h2lds && L.h_aliased()) ? reinterpret_cast<unsigned short*>(lds + L.off_h2())
                                                      : (g_slot ? g_slot + L.hbytes() / 2 : nullptr);
-    const bool learned = PC == 2 && L.learned && P.pc_mode == 2;
+    const bool learned = PC >= 2 && L.learned && P.pc_mode == 2;
     const bool inkernel = learned && P.chunk == 0;  // network evaluated inside this kernel
     constexpr int NW = BLOCK / 64;
     __shared__ float s_red[4][NW];
@@ -186,6 +184,7 @@ __global__ void __launch_bounds__(BLOCK, PC ? 2 : 3) relax_persistent_kernel(Bat
         fs.lastNeg = 0;
         int step = 0;
         V3 Gk = zero3();  // substrate force of this thread's atom (learned gains only)
+        int gkey = -1;    // grid window held in this atom's cache slot (none yet)
         if (P.chunk > 0) {  // resume the suspended state of this system
             const float4 fst = B.fire[s];
             fs.dt = fst.x;
@@ -292,12 +291,13 @@ __global__ void __launch_bounds__(BLOCK, PC ? 2 : 3) relax_persistent_kernel(Bat
                     }
                 }
                 if (P.bGridFF) {
+                    float4* gc = B.gridcache + (size_t)(I.a0 + i) * 16;
                     if (PC == 2) {  // the network's features reuse the substrate force of this step
                         Gk = zero3();
-                        grid_sample(P.grid, pi, mk(plq.x, plq.y, plq.z), Gk);
+                        grid_sample_cached(P.grid, pi, mk(plq.x, plq.y, plq.z), gc, gkey, Gk);
                         f += Gk;
                     } else {
-                        grid_sample(P.grid, pi, mk(plq.x, plq.y, plq.z), f);
+                        grid_sample_cached(P.grid, pi, mk(plq.x, plq.y, plq.z), gc, gkey, f);
                     }
                 }
                 s_fnb[i] = make_float4(f.x, f.y, f.z, 0.f);
@@ -351,7 +351,7 @@ __global__ void __launch_bounds__(BLOCK, PC ? 2 : 3) relax_persistent_kernel(Bat
                                     M, nc, myc, hasAtom, hasNode, ng, bk[0], fv[0], [&] {
                                         if (PC == 2 && inkernel && step % P.pc_every == 0)
                                             propose_gains<BLOCK>(P, B.neighs + I.a0, B.REQs + I.a0, B.elem + I.a0,
-                                                                 s_pos, s_fnb, s_req, s_h, g_h, s_out, s_gain,
+                                                                 s_pos, s_fnb, s_req, s_h, g_h, s_gain,
                                                                  &s_cred[0][0][0], &s_csum[0][0], N, M, nc, myc, plq,
                                                                  Gk);
                                         if (PC == 2 && inkernel && step % P.pc_every == 0 && L.h_aliased()) {
@@ -509,7 +509,9 @@ __global__ void __launch_bounds__(BLOCK, PC ? 2 : 3) relax_persistent_kernel(Bat
     FC_INST(BL, false, 1) \
     FC_INST(BL, true, 1)  \
     FC_INST(BL, false, 2) \
-    FC_INST(BL, true, 2)
+    FC_INST(BL, true, 2)  \
+    FC_INST(BL, false, 3) \
+    FC_INST(BL, true, 3)
 FC_INST2(64)
 FC_INST2(128)
 FC_INST2(192)
@@ -527,11 +529,13 @@ static const void* kernel_ptr(int block) {
         default: return nullptr;
     }
 }
-// pc: preconditioner mode 0 (off), 1 (constant gains), 2 (learned gains)
+// pc: preconditioner mode 0 (off), 1 (constant gains), 2 (learned gains, network
+// inside the kernel), 3 (learned gains read from memory: chunked execution)
 static const void* kernel_ptr(int block, int hb, int pc) {
     switch (pc) {
         case 1: return hb ? kernel_ptr<true, 1>(block) : kernel_ptr<false, 1>(block);
         case 2: return hb ? kernel_ptr<true, 2>(block) : kernel_ptr<false, 2>(block);
+        case 3: return hb ? kernel_ptr<true, 3>(block) : kernel_ptr<false, 3>(block);
         default: return hb ? kernel_ptr<true, 0>(block) : kernel_ptr<false, 0>(block);
     }
 }
@@ -553,7 +557,7 @@ extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups,
     Q.counter = counters;
     Q.lowest = steal ? 0 : cls;
     Q.hslot = hslot;
-    const void* k = fcd::kernel_ptr(block, hb, P->pc_mode);
+    const void* k = fcd::kernel_ptr(block, hb, (P->pc_mode == 2 && P->chunk > 0) ? 3 : P->pc_mode);
     if (!k) return hipErrorInvalidValue;
     void* args[] = {(void*)B, (void*)P, (void*)&Q, (void*)&cls, (void*)&L};
     return hipLaunchKernel(k, dim3(ngroups), dim3(block), args, lds_bytes, stream);

@@ -298,6 +298,7 @@ bool World::upload_all(bool bParams, bool bForces, bool bVel) {
         d_info.upload(info, stream);
         d_node_sys.upload(node_sys, stream);
         d_comp.upload(comp, stream);
+        d_gridcache.alloc(std::max<size_t>((size_t)ntotA * 16, 1));
         d_ncomp.upload(ncomp, stream);
         d_elem.upload(elem, stream);
         d_fneigh.alloc(std::max<size_t>((size_t)ntotM * 8, 1));
@@ -395,6 +396,7 @@ fcd::BatchDev World::batch_dev() {
     B.elem = d_elem.p;
     B.gnode = d_gnode.p;
     B.gcomp = d_gcomp.p;
+    B.gridcache = d_gridcache.p;
     B.snap = (snap_on && d_snap.p) ? d_snap.p : nullptr;
     B.snap_every = std::max(snap_every, 1);
     B.nsnap = nsnap;
@@ -530,7 +532,7 @@ void World::plan_classes() {
         C.cls = c;
         if (C.order.empty()) continue;
         int nb = 1;
-        fc_persistent_occupancy(C.block, hb_batch ? 1 : 0, pc_mode, C.lds, &nb);
+        fc_persistent_occupancy(C.block, hb_batch ? 1 : 0, (pc_mode == 2 && !learned_inkernel) ? 3 : pc_mode, C.lds, &nb);
         nb = std::max(nb, 1);
         C.occ = nb;
         int g = (int)std::lround((double)ncu * nb * C.work / std::max(wtot, 1e-30));

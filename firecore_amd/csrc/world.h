@@ -118,6 +118,7 @@ struct World {
     hipStream_t stream = nullptr;
     std::vector<hipStream_t> class_streams;
     DevArr<float4> d_pcforce;
+    DevArr<float4> d_gridcache;  // per-atom GridFF window cache (persistent kernel)
     DevArr<float4> d_atoms, d_atoms0, d_aforce, d_avel, d_REQs, d_PLQs, d_apars, d_bLs, d_bKs, d_Ksp, d_Kpp, d_fneigh,
         d_fire, d_grid;
     DevArr<int4> d_neighs, d_bkNeighs;
