@@ -966,7 +966,7 @@ int set_graphs(int on) {
 // stealing.  Returns the previous mode, or -1 for an invalid mode.
 int set_sched(int mode) {
     World& W = world();
-    if (mode < 0 || mode > 1) return -1;
+    if (mode < 0 || mode > 2) return -1;
     const int old = W.sched_mode;
     W.sched_mode = mode;
     if (W.dev_ready && W.nSys > 0) W.plan_classes();

@@ -112,6 +112,20 @@ __device__ __forceinline__ float wave_max(float v) {
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
     return v;
 }
+// Wave totals of 4 per-lane values by a reduce-scatter butterfly (7
+// shuffles instead of 4 full reductions of 6): lane l returns the total of
+// value l >> 4.
+__device__ __forceinline__ float wave_sum4(float v0, float v1, float v2, float v3) {
+    const int lane = threadIdx.x & 63;
+    const bool u32 = (lane & 32) != 0;
+    const float a = (u32 ? v2 : v0) + __shfl_xor(u32 ? v0 : v2, 32, 64);
+    const float b = (u32 ? v3 : v1) + __shfl_xor(u32 ? v1 : v3, 32, 64);
+    const bool u16 = (lane & 16) != 0;
+    float s = (u16 ? b : a) + __shfl_xor(u16 ? a : b, 16, 64);
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    return s;
+}
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -139,45 +153,48 @@ __device__ __forceinline__ void precondition(const RunParams& P, const float4* s
         r = xyz(s_pos[tid]) - org;
         s_fnb[tid] = make_float4(fv0.x, fv0.y, fv0.z, 0.f);
     }
-    // per-component sums: n, r, F, r x F, r r^T (one wave reduction at a
-    // time keeps the register pressure of this phase low)
+    // per-component sums: n, r, F, r x F, r r^T; wave totals into s_cred
     {
         const V3 f = fv0;
         const V3 rf = cross(r, f);
         for (int c = 0; c < nc; c++) {
-            const bool mine = hasAtom && myc == c;
-#pragma unroll 1
-            for (int k = 0; k < PC_NSUM; k++) {
-                float x;
-                switch (k) {
-                    case 0: x = 1.f; break;
-                    case 1: x = r.x; break;
-                    case 2: x = r.y; break;
-                    case 3: x = r.z; break;
-                    case 4: x = f.x; break;
-                    case 5: x = f.y; break;
-                    case 6: x = f.z; break;
-                    case 7: x = rf.x; break;
-                    case 8: x = rf.y; break;
-                    case 9: x = rf.z; break;
-                    case 10: x = r.x * r.x; break;
-                    case 11: x = r.y * r.y; break;
-                    case 12: x = r.z * r.z; break;
-                    case 13: x = r.x * r.y; break;
-                    case 14: x = r.x * r.z; break;
-                    default: x = r.y * r.z; break;
-                }
-                const float s = wave_sum(mine ? x : 0.f);
-                if (lane == 0) s_cred[(wid * PC_MAXC + c) * PC_NSUM + k] = s;
-            }
+            const float m = (hasAtom && myc == c) ? 1.f : 0.f;
+            float* dst = s_cred + (wid * PC_MAXC + c) * PC_NSUM + (lane >> 4);
+            const float t0 = wave_sum4(m, m * r.x, m * r.y, m * r.z);
+            if ((lane & 15) == 0) dst[0] = t0;
+            const float t1 = wave_sum4(m * f.x, m * f.y, m * f.z, m * rf.x);
+            if ((lane & 15) == 0) dst[4] = t1;
+            const float t2 = wave_sum4(m * rf.y, m * rf.z, m * r.x * r.x, m * r.y * r.y);
+            if ((lane & 15) == 0) dst[8] = t2;
+            const float t3 = wave_sum4(m * r.z * r.z, m * r.x * r.y, m * r.x * r.z, m * r.y * r.z);
+            if ((lane & 15) == 0) dst[12] = t3;
         }
     }
     __syncthreads();
-    if (tid < nc * PC_NSUM) {
-        float s = 0.f;
+    // one thread per component: block totals and the rigid-body fit, kept in
+    // s_csum[c*16]: count, centroid (1..3), mean force t (4..6), angular rate w (7..9)
+    if (tid < nc) {
+        float S[PC_NSUM];
 #pragma unroll
-        for (int w = 0; w < NW; w++) s += s_cred[w * PC_MAXC * PC_NSUM + tid];
-        s_csum[tid] = s;
+        for (int k = 0; k < PC_NSUM; k++) {
+            float v = 0.f;
+#pragma unroll
+            for (int w = 0; w < NW; w++) v += s_cred[(w * PC_MAXC + tid) * PC_NSUM + k];
+            S[k] = v;
+        }
+        V3 cen, t, w;
+        comp_rigid(S, cen, t, w);
+        float* o = s_csum + tid * PC_NSUM;
+        o[0] = S[0];
+        o[1] = cen.x;
+        o[2] = cen.y;
+        o[3] = cen.z;
+        o[4] = t.x;
+        o[5] = t.y;
+        o[6] = t.z;
+        o[7] = w.x;
+        o[8] = w.y;
+        o[9] = w.z;
     }
     // learned gains are refreshed here (exact atom forces are in s_fnb; the
     // recoil areas s_fself/s_fng are free until the star step below)
@@ -218,8 +235,8 @@ __device__ __forceinline__ void precondition(const RunParams& P, const float4* s
             if (b4[k] < 0) break;
             f += xyz(s_fng[b4[k]]);
         }
-        V3 cen, t, w;
-        comp_rigid(s_csum + myc * PC_NSUM, cen, t, w);
+        const float* o = s_csum + myc * PC_NSUM;
+        const V3 cen = mk(o[1], o[2], o[3]), t = mk(o[4], o[5], o[6]), w = mk(o[7], o[8], o[9]);
         const float gc = learned ? s_gain[M + myc] : P.pc_grigid;
         f += (t + cross(w, r - cen)) * gc;
         fv0 = f;

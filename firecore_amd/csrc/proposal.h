@@ -320,15 +320,14 @@ __device__ __forceinline__ void mlp_feature_row(const RunParams& P, const float4
 }
 
 // Network evaluation inside the relaxation: features from the exact forces in
-// s_F and the substrate force Gi of the calling thread's atom (sampled in the
-// same step), forward pass, gains into s_gain (per node, then per component at
-// s_gain[M + c]).  Uses s_cred (component partial sums) and s_csum[c*16] (the
-// component's atom count) from precondition().  Every thread calls it.
+// s_F (the substrate force of each atom is sampled again here), forward pass, gains into s_gain (per node, then per component at
+// s_gain[M + c]).  s_cred receives per-wave partial sums (component output
+// sums and atom counts).  Every thread calls it.
 template <int BLOCK>
 __device__ __forceinline__ void propose_gains(const RunParams& P, const int4* nb, const float4* req, const int* elem,
                               const float4* s_pos, const float4* s_F, float4* s_req, unsigned short* h,
-                              unsigned short* hG, float* s_gain, float* s_cred, const float* s_csum,
-                              int N, int M, int nc, int myc, float4 plq, V3 Gi) {
+                              unsigned short* hG, float* s_gain, float* s_cred, int N, int M, int nc, int myc,
+                              float4 plq) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wid = tid >> 6;
@@ -337,7 +336,7 @@ __device__ __forceinline__ void propose_gains(const RunParams& P, const int4* nb
     // them in LDS over the (constant) REQ rows, restored from req below
     int4* s_nb = reinterpret_cast<int4*>(s_req);
     if (tid < N) {
-        mlp_feature_row(P, s_pos, s_F, nb, elem, N, M, tid, plq, h + tid * MLP_HS, &Gi);
+        mlp_feature_row(P, s_pos, s_F, nb, elem, N, M, tid, plq, h + tid * MLP_HS);
         s_nb[tid] = nb[tid];
     }
     __syncthreads();
@@ -345,17 +344,35 @@ __device__ __forceinline__ void propose_gains(const RunParams& P, const int4* nb
     if (tid < N) s_req[tid] = req[tid];
     if (tid < M) s_gain[tid] = P.pc_gstar * sigmoidf_(o.x);
     for (int c = 0; c < nc; c++) {
-        const float v = wave_sum((tid < N && myc == c) ? o.y : 0.f);
-        if (lane == 0) s_cred[(wid * PC_MAXC + c) * PC_NSUM] = v;
+        const bool mine = tid < N && myc == c;
+        const float v = wave_sum(mine ? o.y : 0.f);
+        const float n = wave_sum(mine ? 1.f : 0.f);
+        if (lane == 0) {
+            s_cred[(wid * PC_MAXC + c) * PC_NSUM] = v;
+            s_cred[(wid * PC_MAXC + c) * PC_NSUM + 1] = n;
+        }
     }
     __syncthreads();
     if (tid < nc) {
-        float s = 0.f;
+        float s = 0.f, n = 0.f;
 #pragma unroll
-        for (int w = 0; w < NW; w++) s += s_cred[(w * PC_MAXC + tid) * PC_NSUM];
-        s_gain[M + tid] = P.pc_grigid * sigmoidf_(s / fmaxf(s_csum[tid * PC_NSUM], 1.f));
+        for (int w = 0; w < NW; w++) {
+            s += s_cred[(w * PC_MAXC + tid) * PC_NSUM];
+            n += s_cred[(w * PC_MAXC + tid) * PC_NSUM + 1];
+        }
+        s_gain[M + tid] = P.pc_grigid * sigmoidf_(s / fmaxf(n, 1.f));
     }
     __syncthreads();
+}
+
+// Out-of-line entry of propose_gains for the relaxation kernel: a call keeps
+// the network's register demand out of the caller's allocation.
+template <int BLOCK>
+__device__ __noinline__ void propose_gains_call(const RunParams& P, const int4* nb, const float4* req, const int* elem,
+                                                const float4* s_pos, const float4* s_F, float4* s_req,
+                                                unsigned short* h, unsigned short* hG, float* s_gain, float* s_cred,
+                                                int N, int M, int nc, int myc, float4 plq) {
+    propose_gains<BLOCK>(P, nb, req, elem, s_pos, s_F, s_req, h, hG, s_gain, s_cred, N, M, nc, myc, plq);
 }
 
 }  // namespace fcd

@@ -83,7 +83,7 @@ struct Queues {
 };
 
 template <int BLOCK, bool HB, int PC>
-__global__ void __launch_bounds__(BLOCK, PC >= 2 ? 2 : 3) relax_persistent_kernel(BatchDev B, RunParams P, Queues Q, int cls,
+__global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, RunParams P, Queues Q, int cls,
                                                                  LdsLayout L) {
     extern __shared__ float4 lds_dyn[];
     char* lds = reinterpret_cast<char*>(lds_dyn);
@@ -145,45 +145,17 @@ __global__ void __launch_bounds__(BLOCK, PC >= 2 ? 2 : 3) relax_persistent_kerne
         // transposed to [word][atom] so that lanes read consecutive banks
         for (int w = tid; w < N * W; w += BLOCK) s_mask[(w % W) * N + (w / W)] = B.excl_rot[I.mask0 + w];
 
-        // ---- per-thread persistent state: atom tid, node tid, vecs tid and tid+BLOCK
         const bool hasAtom = tid < N;
         const bool hasNode = tid < M;
-        float4 plq = hasAtom ? B.PLQs[I.a0 + tid] : make_float4(0, 0, 0, 0);
-        const int myc = (PC && hasAtom) ? B.comp[I.a0 + tid] : 0;
         const int nc = PC ? B.ncomp[s] : 0;
-        int4 ng = make_int4(-1, -1, -1, -1);
-        float4 apar = make_float4(0, 0, 0, 0), bL = apar, bK = apar, ksp = apar, kpp = apar;
-        if (hasNode) {
-            ng = B.neighs[I.a0 + tid];
-            apar = B.apars[I.n0 + tid];
-            bL = B.bLs[I.n0 + tid];
-            bK = B.bKs[I.n0 + tid];
-            ksp = B.Ksp[I.n0 + tid];
-            kpp = B.Kpp[I.n0 + tid];
-        }
-        int4 bk[2];
         V3 vel[2];
 #pragma unroll
-        for (int r = 0; r < 2; r++) {
-            const int v = tid + r * BLOCK;
-            bk[r] = make_int4(-1, -1, -1, -1);
-            vel[r] = zero3();
-            if (v < V) {
-                // back-neighbours as local indices into s_fng (sigma rows [0,4M), pi rows [4M,8M))
-                int4 b = B.bkNeighs[I.v0 + v];
-                const int base = I.n0 * 8;
-                bk[r].x = b.x >= 0 ? b.x - base : -1;
-                bk[r].y = b.y >= 0 ? b.y - base : -1;
-                bk[r].z = b.z >= 0 ? b.z - base : -1;
-                bk[r].w = b.w >= 0 ? b.w - base : -1;
-            }
-        }
+        for (int r = 0; r < 2; r++) vel[r] = zero3();
         FireState fs;
         fs.dt = P.fire.dt_max;
         fs.damping = P.fire.damp_max;
         fs.lastNeg = 0;
         int step = 0;
-        V3 Gk = zero3();  // substrate force of this thread's atom (learned gains only)
         int gkey = -1;    // grid window held in this atom's cache slot (none yet)
         if (P.chunk > 0) {  // resume the suspended state of this system
             const float4 fst = B.fire[s];
@@ -208,6 +180,42 @@ __global__ void __launch_bounds__(BLOCK, PC >= 2 ? 2 : 3) relax_persistent_kerne
         int status = ST_RUNNING;
         float fmax2_last = 0.f;
         V3 fv[2];
+        // The learned form evaluates the network between segments of the step
+        // loop (one segment per proposal period): the step loop breaks out at a
+        // proposal step, the network runs with the velocities parked in memory,
+        // and the segment restarts that step.  Nothing of the step loop is live
+        // across the network, so neither part adds to the other's registers.
+        bool fresh = false;  // gains of the current step already proposed
+        for (;;) {
+        bool propose_now = false;
+        // ---- per-thread state of this segment: atom tid, node tid, vecs tid and tid+BLOCK
+        const float4 plq = hasAtom ? B.PLQs[I.a0 + tid] : make_float4(0, 0, 0, 0);
+        const int myc = (PC && hasAtom) ? B.comp[I.a0 + tid] : 0;
+        int4 ng = make_int4(-1, -1, -1, -1);
+        float4 apar = make_float4(0, 0, 0, 0), bL = apar, bK = apar, ksp = apar, kpp = apar;
+        if (hasNode) {
+            ng = B.neighs[I.a0 + tid];
+            apar = B.apars[I.n0 + tid];
+            bL = B.bLs[I.n0 + tid];
+            bK = B.bKs[I.n0 + tid];
+            ksp = B.Ksp[I.n0 + tid];
+            kpp = B.Kpp[I.n0 + tid];
+        }
+        int4 bk[2];
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int v = tid + r * BLOCK;
+            bk[r] = make_int4(-1, -1, -1, -1);
+            if (v < V) {
+                // back-neighbours as local indices into s_fng (sigma rows [0,4M), pi rows [4M,8M))
+                int4 b = B.bkNeighs[I.v0 + v];
+                const int base = I.n0 * 8;
+                bk[r].x = b.x >= 0 ? b.x - base : -1;
+                bk[r].y = b.y >= 0 ? b.y - base : -1;
+                bk[r].z = b.z >= 0 ? b.z - base : -1;
+                bk[r].w = b.w >= 0 ? b.w - base : -1;
+            }
+        }
         for (;;) {
             if (step >= step_end) {  // chunk done: suspend (state written back below)
                 suspended = true;
@@ -292,13 +300,7 @@ __global__ void __launch_bounds__(BLOCK, PC >= 2 ? 2 : 3) relax_persistent_kerne
                 }
                 if (P.bGridFF) {
                     float4* gc = B.gridcache + (size_t)(I.a0 + i) * 16;
-                    if (PC == 2) {  // the network's features reuse the substrate force of this step
-                        Gk = zero3();
-                        grid_sample_cached(P.grid, pi, mk(plq.x, plq.y, plq.z), gc, gkey, Gk);
-                        f += Gk;
-                    } else {
-                        grid_sample_cached(P.grid, pi, mk(plq.x, plq.y, plq.z), gc, gkey, f);
-                    }
+                    grid_sample_cached(P.grid, pi, mk(plq.x, plq.y, plq.z), gc, gkey, f);
                 }
                 s_fnb[i] = make_float4(f.x, f.y, f.z, 0.f);
             }
@@ -346,20 +348,14 @@ __global__ void __launch_bounds__(BLOCK, PC >= 2 ? 2 : 3) relax_persistent_kerne
                     fv[r] = f;
                 }
             }
+            if (PC == 2 && inkernel && !fresh && step % P.pc_every == 0) {  // uniform across the block
+                if (hasAtom) s_fnb[tid] = make_float4(fv[0].x, fv[0].y, fv[0].z, 0.f);
+                propose_now = true;
+                break;
+            }
             if (PC)
                 precondition<BLOCK>(P, s_pos, s_fnb, s_fself, s_fng, &s_cred[0][0][0], &s_csum[0][0], s_gain, learned, N,
-                                    M, nc, myc, hasAtom, hasNode, ng, bk[0], fv[0], [&] {
-                                        if (PC == 2 && inkernel && step % P.pc_every == 0)
-                                            propose_gains<BLOCK>(P, B.neighs + I.a0, B.REQs + I.a0, B.elem + I.a0,
-                                                                 s_pos, s_fnb, s_req, s_h, g_h, s_gain,
-                                                                 &s_cred[0][0][0], &s_csum[0][0], N, M, nc, myc, plq,
-                                                                 Gk);
-                                        if (PC == 2 && inkernel && step % P.pc_every == 0 && L.h_aliased()) {
-                                            for (int w = tid; w < N * W; w += BLOCK)
-                                                s_mask[(w % W) * N + (w / W)] = B.excl_rot[I.mask0 + w];
-                                            __syncthreads();
-                                        }
-                                    });
+                                    M, nc, myc, hasAtom, hasNode, ng, bk[0], fv[0], [] {});
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const int v = tid + r * BLOCK;
@@ -479,7 +475,30 @@ __global__ void __launch_bounds__(BLOCK, PC >= 2 ? 2 : 3) relax_persistent_kerne
                 }
             }
             step++;
+            fresh = false;
             __syncthreads();
+        }
+        if (!propose_now) break;
+        // ---- network evaluation between segments (learned form)
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int v = tid + r * BLOCK;
+            if (v < V) B.avel[I.v0 + v] = make_float4(vel[r].x, vel[r].y, vel[r].z, 0.f);
+        }
+        __syncthreads();
+        propose_gains<BLOCK>(P, B.neighs + I.a0, B.REQs + I.a0, B.elem + I.a0, s_pos, s_fnb, s_req, s_h, g_h,
+                             s_gain, &s_cred[0][0][0], N, M, nc, hasAtom ? B.comp[I.a0 + tid] : 0,
+                             hasAtom ? B.PLQs[I.a0 + tid] : make_float4(0, 0, 0, 0));
+        if (L.h_aliased()) {  // the exclusion rows were overwritten by the activations
+            for (int w = tid; w < N * W; w += BLOCK) s_mask[(w % W) * N + (w / W)] = B.excl_rot[I.mask0 + w];
+        }
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+            const int v = tid + r * BLOCK;
+            if (v < V) vel[r] = xyz(B.avel[I.v0 + v]);
+        }
+        fresh = true;
+        __syncthreads();
         }
         (void)fmax2_last;
         if (suspended && tid == 0) {

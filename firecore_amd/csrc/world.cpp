@@ -537,6 +537,9 @@ void World::plan_classes() {
         C.occ = nb;
         int g = (int)std::lround((double)ncu * nb * C.work / std::max(wtot, 1e-30));
         if (sched_mode == 1) g = ncu * nb;  // every class may fill the device; the dispatcher interleaves them
+        // twice the resident width: the extra workgroups are dispatched as
+        // other classes retire and drain the remaining queue
+        if (sched_mode == 2) g = 2 * ncu * nb;
         C.ngroups = std::max(1, std::min(g, (int)C.order.size()));
         classes.push_back(C);
     }
