@@ -234,26 +234,38 @@ __device__ __forceinline__ void mlp_layer_lean(const unsigned short* hin, const 
     constexpr int KS = (KIN == 16) ? 1 : 4;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int nrb = (n + 15) >> 4;
-    // a wave takes whole row blocks: the A operand of a row block (its own
-    // rows and the neighbour sums) is built once and reused for the four
-    // column blocks; the weight fragments come from L1/L2
+    // the weight fragments of all four column blocks are loaded once per layer
+    // (one memory latency); a wave then takes whole row blocks, builds their A
+    // operand (own rows and neighbour sums) once and runs four independent
+    // accumulation chains
+    bf16x4 bw[4][NK];
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+        for (int kc = 0; kc < NK; kc++) bw[cb][kc] = mlp_b_frag<KIN>(W, cb, kc, lane);
+    float bc[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++) bc[cb] = bias[cb * 16 + (lane & 15)];
 #pragma unroll 1
     for (int rb = wid; rb < nrb; rb += NW) {
         bf16x4 a[NK];
 #pragma unroll
         for (int kc = 0; kc < NK; kc++) a[kc] = mlp_a_frag<KS>(hin, nb, n, rb, kc, lane);
-#pragma unroll 1
-        for (int cb = 0; cb < 4; cb++) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        f32x4 acc[4];
 #pragma unroll
-            for (int kc = 0; kc < NK; kc++)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kc], mlp_b_frag<KIN>(W, cb, kc, lane), acc, 0, 0, 0);
+        for (int cb = 0; cb < 4; cb++) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < NK; kc++)
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++)
+                acc[cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kc], bw[cb][kc], acc[cb], 0, 0, 0);
+#pragma unroll
+        for (int cb = 0; cb < 4; cb++) {
             const int col = cb * 16 + (lane & 15);
-            const float bc = bias[col];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int row = rb * 16 + 4 * (lane >> 4) + i;
-                if (row < n) hout[row * MLP_HS + col] = f2bf(silu(acc[i] + bc));
+                if (row < n) hout[row * MLP_HS + col] = f2bf(silu(acc[cb][i] + bc[cb]));
             }
         }
     }
