@@ -115,11 +115,13 @@ def roofline(r, learned, every=PROP_EVERY):
     """Roofline of the dominant kernel (the size class with the longest launch).
 
     The persistent kernel keeps every system's state in LDS/registers; its
-    memory traffic is the B-spline GridFF gathers, 64 float4 coefficients
-    (1024 B) per atom and step from a 10 MB grid that does not stay in the
-    per-XCD L2.  achieved = those algorithmic gather bytes of the launch over
-    its measured average duration (HIP events on the launch's stream), against
-    the HBM peak; the FP32 work of the same launch is reported beside it.
+    algorithmic memory traffic is the B-spline GridFF gathers, 64 float4
+    coefficients (1024 B) per atom and step from a 10 MB grid that does not
+    stay in the per-XCD L2 (a per-atom window cache serves most of them, so
+    the measured traffic is lower).  achieved = those algorithmic gather bytes
+    of the launch over its measured average duration (HIP events on the
+    launch's stream), against the HBM peak; the FP32 work of the same launch
+    is reported beside it.
     traffic = L2-to-fabric bytes per launch measured with rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated; profiles/), when recorded."""
     B, steps, classes = r["B"], r["nsteps"], r["classes"]
