@@ -219,10 +219,9 @@ __device__ __forceinline__ void mlp_forward(const MlpDev& M, unsigned short* h, 
 }
 
 // ---------------------------------------------------------------------------
-// Register-lean forward pass for use inside the relaxation kernel: every wave
-// computes one 16x16 output tile at a time (4 accumulator registers and the
-// weight fragments of the current column block), so the network adds little
-// register pressure to the force loop.  Activations alternate between two
+// Forward pass used by the relaxation kernel between segments of its step
+// loop (nothing of the force loop is live there, so a layer may hold all its
+// weight fragments in registers).  Activations alternate between two
 // row buffers of n rows, hA (LDS) and hG (LDS when the workgroup's LDS budget
 // allows, else a per-workgroup slot in global memory that stays in L2):
 // layer 1 hA -> hG, layer 2 hG -> hA, layer 3 hA -> hG, head from hG.
@@ -332,9 +331,10 @@ __device__ __forceinline__ void mlp_feature_row(const RunParams& P, const float4
 }
 
 // Network evaluation inside the relaxation: features from the exact forces in
-// s_F (the substrate force of each atom is sampled again here), forward pass, gains into s_gain (per node, then per component at
-// s_gain[M + c]).  s_cred receives per-wave partial sums (component output
-// sums and atom counts).  Every thread calls it.
+// s_F (the substrate force of each atom is sampled again here), forward pass,
+// gains into s_gain (per node, then per component at s_gain[M + c]).  s_cred
+// receives per-wave partial sums (component output sums and atom counts).
+// Every thread calls it; nothing of the step loop is live across the call.
 template <int BLOCK>
 __device__ __forceinline__ void propose_gains(const RunParams& P, const int4* nb, const float4* req, const int* elem,
                               const float4* s_pos, const float4* s_F, float4* s_req, unsigned short* h,
@@ -375,16 +375,6 @@ __device__ __forceinline__ void propose_gains(const RunParams& P, const int4* nb
         s_gain[M + tid] = P.pc_grigid * sigmoidf_(s / fmaxf(n, 1.f));
     }
     __syncthreads();
-}
-
-// Out-of-line entry of propose_gains for the relaxation kernel: a call keeps
-// the network's register demand out of the caller's allocation.
-template <int BLOCK>
-__device__ __noinline__ void propose_gains_call(const RunParams& P, const int4* nb, const float4* req, const int* elem,
-                                                const float4* s_pos, const float4* s_F, float4* s_req,
-                                                unsigned short* h, unsigned short* hG, float* s_gain, float* s_cred,
-                                                int N, int M, int nc, int myc, float4 plq) {
-    propose_gains<BLOCK>(P, nb, req, elem, s_pos, s_F, s_req, h, hG, s_gain, s_cred, N, M, nc, myc, plq);
 }
 
 }  // namespace fcd

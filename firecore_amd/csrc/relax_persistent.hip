@@ -187,318 +187,318 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
         // across the network, so neither part adds to the other's registers.
         bool fresh = false;  // gains of the current step already proposed
         for (;;) {
-        bool propose_now = false;
-        // ---- per-thread state of this segment: atom tid, node tid, vecs tid and tid+BLOCK
-        const float4 plq = hasAtom ? B.PLQs[I.a0 + tid] : make_float4(0, 0, 0, 0);
-        const int myc = (PC && hasAtom) ? B.comp[I.a0 + tid] : 0;
-        int4 ng = make_int4(-1, -1, -1, -1);
-        float4 apar = make_float4(0, 0, 0, 0), bL = apar, bK = apar, ksp = apar, kpp = apar;
-        if (hasNode) {
-            ng = B.neighs[I.a0 + tid];
-            apar = B.apars[I.n0 + tid];
-            bL = B.bLs[I.n0 + tid];
-            bK = B.bKs[I.n0 + tid];
-            ksp = B.Ksp[I.n0 + tid];
-            kpp = B.Kpp[I.n0 + tid];
-        }
-        int4 bk[2];
-#pragma unroll
-        for (int r = 0; r < 2; r++) {
-            const int v = tid + r * BLOCK;
-            bk[r] = make_int4(-1, -1, -1, -1);
-            if (v < V) {
-                // back-neighbours as local indices into s_fng (sigma rows [0,4M), pi rows [4M,8M))
-                int4 b = B.bkNeighs[I.v0 + v];
-                const int base = I.n0 * 8;
-                bk[r].x = b.x >= 0 ? b.x - base : -1;
-                bk[r].y = b.y >= 0 ? b.y - base : -1;
-                bk[r].z = b.z >= 0 ? b.z - base : -1;
-                bk[r].w = b.w >= 0 ? b.w - base : -1;
-            }
-        }
-        for (;;) {
-            if (step >= step_end) {  // chunk done: suspend (state written back below)
-                suspended = true;
-                break;
-            }
-            if (B.snap && step % B.snap_every == 0 && step / B.snap_every < B.nsnap) {
-                float4* dst = B.snap + (size_t)(step / B.snap_every) * B.ntotV + I.v0;
-                for (int v = tid; v < V; v += BLOCK) dst[v] = s_pos[v];
-            }
-            // ================= A: forces (no energies inside the step loop)
-            // Half-ring pair schedule (Newton's third law): thread i takes the
-            // partners j = (i+k) mod N, k = 1..N/2, so every unordered pair is
-            // computed once.  The reaction -f_ij goes to the wave's private
-            // accumulator row: within one instruction the lanes of a wave hold
-            // distinct j, and no other wave writes that row.
-            const int nwAct = (N + 63) >> 6;
-            float4* facc = s_facc + wid * L.nmax;
-            if (wid < nwAct)
-                for (int j = lane; j < N; j += 64) facc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (hasAtom) {
-                const int i = tid;
-                const V3 pi = xyz(s_pos[i]);
-                const float4 ri = s_req[i];
-                const float kQi = COULOMB_K * ri.z;
-                V3 f = zero3();
-                if (P.bNonBonded) {
-                    const int K = (N - 1) >> 1;
-                    for (int w = 0; (w << 5) <= K; w++) {
-                        const unsigned m = s_mask[w * N + i];
-                        const int k0 = max(1, w << 5);
-                        const int k1 = min(K, (w << 5) + 31);
-#pragma unroll 4
-                        for (int k = k0; k <= k1; k++) {
-                            int j = i + k;
-                            j = (j >= N) ? j - N : j;
-                            const float4 pj = s_pos[j];
-                            const float4 rj = s_req[j];
-                            const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                            float fr, ep;
-                            lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
-                            fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
-                            f.x -= d.x * fr;
-                            f.y -= d.y * fr;
-                            f.z -= d.z * fr;
-                            float4 a = facc[j];
-                            a.x += d.x * fr;
-                            a.y += d.y * fr;
-                            a.z += d.z * fr;
-                            facc[j] = a;
-                        }
-                    }
-                    if (!(N & 1)) {  // k = N/2 pairs (i, i+N/2): taken by the lower half only
-                        const int k = N >> 1;
-                        if (i < k) {
-                            const int j = i + k;
-                            const unsigned m = s_mask[(k >> 5) * N + i];
-                            const float4 pj = s_pos[j];
-                            const float4 rj = s_req[j];
-                            const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                            float fr, ep;
-                            lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
-                            fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
-                            f -= d * fr;
-                            float4 a = facc[j];
-                            a.x += d.x * fr;
-                            a.y += d.y * fr;
-                            a.z += d.z * fr;
-                            facc[j] = a;
-                        }
-                    }
-                    // weighted corrections (ring topologies; empty for trees)
-                    for (int c = 0; c < I.ncorr; c++) {
-                        const PairCorrDev pc = B.corr[I.corr0 + c];
-                        if (pc.i != i) continue;
-                        const float4 pj = s_pos[pc.j];
-                        const float4 rj = s_req[pc.j];
-                        const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                        float fr, ep;
-                        lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
-                        f -= d * (fr * pc.w);
-                    }
-                }
-                if (P.bGridFF) {
-                    float4* gc = B.gridcache + (size_t)(I.a0 + i) * 16;
-                    grid_sample_cached(P.grid, pi, mk(plq.x, plq.y, plq.z), gc, gkey, f);
-                }
-                s_fnb[i] = make_float4(f.x, f.y, f.z, 0.f);
-            }
+            bool propose_now = false;
+            // ---- per-thread state of this segment: atom tid, node tid, vecs tid and tid+BLOCK
+            const float4 plq = hasAtom ? B.PLQs[I.a0 + tid] : make_float4(0, 0, 0, 0);
+            const int myc = (PC && hasAtom) ? B.comp[I.a0 + tid] : 0;
+            int4 ng = make_int4(-1, -1, -1, -1);
+            float4 apar = make_float4(0, 0, 0, 0), bL = apar, bK = apar, ksp = apar, kpp = apar;
             if (hasNode) {
-                V3 fbs[4], fps[4], fa, fpi;
-                auto pos = [&](int v) { return xyz(s_pos[v]); };
-                eval_node(tid, N, M, pos, ng, apar, bL, bK, ksp, kpp, P.sw, fbs, fps, fa, fpi);
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    s_fng[tid * 4 + k] = make_float4(fbs[k].x, fbs[k].y, fbs[k].z, 0.f);
-                    s_fng[4 * M + tid * 4 + k] = make_float4(fps[k].x, fps[k].y, fps[k].z, 0.f);
-                }
-                s_fself[tid] = make_float4(fa.x, fa.y, fa.z, 0.f);
-                s_fself[M + tid] = make_float4(fpi.x, fpi.y, fpi.z, 0.f);
+                ng = B.neighs[I.a0 + tid];
+                apar = B.apars[I.n0 + tid];
+                bL = B.bLs[I.n0 + tid];
+                bK = B.bKs[I.n0 + tid];
+                ksp = B.Ksp[I.n0 + tid];
+                kpp = B.Kpp[I.n0 + tid];
             }
-            __syncthreads();
-
-            // ================= B: assemble + reductions
-            float vf = 0.f, vv = 0.f, ff = 0.f, fm2 = 0.f;
+            int4 bk[2];
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const int v = tid + r * BLOCK;
-                fv[r] = zero3();
+                bk[r] = make_int4(-1, -1, -1, -1);
                 if (v < V) {
-                    V3 f = zero3();
-                    const bool isPi = v >= N;
-                    if (!isPi) {
-                        f = xyz(s_fnb[v]);
-                        for (int w = 0; w < nwAct; w++) f += xyz(s_facc[w * L.nmax + v]);
-                        if (v < M) f += xyz(s_fself[v]);
-                    } else {
-                        f = xyz(s_fself[M + (v - N)]);
-                    }
-                    const int b4[4] = {bk[r].x, bk[r].y, bk[r].z, bk[r].w};
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        if (b4[k] < 0) break;
-                        f += xyz(s_fng[b4[k]]);
-                    }
-                    if (isPi) {
-                        const V3 p = xyz(s_pos[v]);
-                        f -= p * dot(p, f);
-                    }
-                    fm2 = fmaxf(fm2, dot(f, f));  // convergence always on the exact force
-                    fv[r] = f;
+                    // back-neighbours as local indices into s_fng (sigma rows [0,4M), pi rows [4M,8M))
+                    int4 b = B.bkNeighs[I.v0 + v];
+                    const int base = I.n0 * 8;
+                    bk[r].x = b.x >= 0 ? b.x - base : -1;
+                    bk[r].y = b.y >= 0 ? b.y - base : -1;
+                    bk[r].z = b.z >= 0 ? b.z - base : -1;
+                    bk[r].w = b.w >= 0 ? b.w - base : -1;
                 }
             }
-            if (PC == 2 && inkernel && !fresh && step % P.pc_every == 0) {  // uniform across the block
-                if (hasAtom) s_fnb[tid] = make_float4(fv[0].x, fv[0].y, fv[0].z, 0.f);
-                propose_now = true;
-                break;
-            }
-            if (PC)
-                precondition<BLOCK>(P, s_pos, s_fnb, s_fself, s_fng, &s_cred[0][0][0], &s_csum[0][0], s_gain, learned, N,
-                                    M, nc, myc, hasAtom, hasNode, ng, bk[0], fv[0], [] {});
-#pragma unroll
-            for (int r = 0; r < 2; r++) {
-                const int v = tid + r * BLOCK;
-                if (v < V) {
-                    V3 f = fv[r];
-                    const float f2 = dot(f, f);
-                    if (P.Flimit > 0.f && f2 > P.Flimit * P.Flimit) f = f * (P.Flimit / sqrtf(f2));
-                    fv[r] = f;
-                    vf += dot(vel[r], f);
-                    vv += dot(vel[r], vel[r]);
-                    ff += dot(f, f);
+            for (;;) {
+                if (step >= step_end) {  // chunk done: suspend (state written back below)
+                    suspended = true;
+                    break;
                 }
-            }
-            vf = wave_sum(vf);
-            vv = wave_sum(vv);
-            ff = wave_sum(ff);
-            fm2 = wave_max(fm2);
-            if (lane == 0) {
-                s_red[0][wid] = vf;
-                s_red[1][wid] = vv;
-                s_red[2][wid] = ff;
-                s_red[3][wid] = fm2;
-            }
-            __syncthreads();
-            vf = s_red[0][0];
-            vv = s_red[1][0];
-            ff = s_red[2][0];
-            fm2 = s_red[3][0];
-#pragma unroll
-            for (int w = 1; w < NW; w++) {
-                vf += s_red[0][w];
-                vv += s_red[1][w];
-                ff += s_red[2][w];
-                fm2 = fmaxf(fm2, s_red[3][w]);
-            }
-            fmax2_last = fm2;
-            const float crit = (P.conv_mode == CONV_F2SUM) ? ff : fm2;
-            if (!(ff == ff) || isinf(ff)) status = ST_NAN;
-            else if (crit < P.Fconv2) status = ST_CONVERGED;
-            else if (step >= P.nstep_max) status = ST_MAXSTEP;
-            if (status != ST_RUNNING) {
-                // energy of this geometry: precise 1/r, FP64 accumulation and reduction
-                double epart = 0.0;
-                if (hasAtom && P.bNonBonded) {
+                if (B.snap && step % B.snap_every == 0 && step / B.snap_every < B.nsnap) {
+                    float4* dst = B.snap + (size_t)(step / B.snap_every) * B.ntotV + I.v0;
+                    for (int v = tid; v < V; v += BLOCK) dst[v] = s_pos[v];
+                }
+                // ================= A: forces (no energies inside the step loop)
+                // Half-ring pair schedule (Newton's third law): thread i takes the
+                // partners j = (i+k) mod N, k = 1..N/2, so every unordered pair is
+                // computed once.  The reaction -f_ij goes to the wave's private
+                // accumulator row: within one instruction the lanes of a wave hold
+                // distinct j, and no other wave writes that row.
+                const int nwAct = (N + 63) >> 6;
+                float4* facc = s_facc + wid * L.nmax;
+                if (wid < nwAct)
+                    for (int j = lane; j < N; j += 64) facc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (hasAtom) {
                     const int i = tid;
                     const V3 pi = xyz(s_pos[i]);
                     const float4 ri = s_req[i];
                     const float kQi = COULOMB_K * ri.z;
-                    double e = 0.0;  // each unordered pair once (half ring)
-                    const int Kh = N >> 1;
-                    for (int k = 1; k <= Kh; k++) {
-                        if (!(N & 1) && k == Kh && i >= Kh) break;
-                        if ((s_mask[(k >> 5) * N + i] >> (k & 31)) & 1u) continue;
-                        int j = i + k;
-                        j = (j >= N) ? j - N : j;
-                        const float4 pj = s_pos[j];
-                        const float4 rj = s_req[j];
-                        float fr, ep;
-                        lj_pair<true, true>(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), ri.x + rj.x, ri.y * rj.y,
-                                            kQi * rj.z, ri.w * rj.w, fr, ep);
-                        e += (double)ep;
+                    V3 f = zero3();
+                    if (P.bNonBonded) {
+                        const int K = (N - 1) >> 1;
+                        for (int w = 0; (w << 5) <= K; w++) {
+                            const unsigned m = s_mask[w * N + i];
+                            const int k0 = max(1, w << 5);
+                            const int k1 = min(K, (w << 5) + 31);
+#pragma unroll 4
+                            for (int k = k0; k <= k1; k++) {
+                                int j = i + k;
+                                j = (j >= N) ? j - N : j;
+                                const float4 pj = s_pos[j];
+                                const float4 rj = s_req[j];
+                                const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                                float fr, ep;
+                                lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                                fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
+                                f.x -= d.x * fr;
+                                f.y -= d.y * fr;
+                                f.z -= d.z * fr;
+                                float4 a = facc[j];
+                                a.x += d.x * fr;
+                                a.y += d.y * fr;
+                                a.z += d.z * fr;
+                                facc[j] = a;
+                            }
+                        }
+                        if (!(N & 1)) {  // k = N/2 pairs (i, i+N/2): taken by the lower half only
+                            const int k = N >> 1;
+                            if (i < k) {
+                                const int j = i + k;
+                                const unsigned m = s_mask[(k >> 5) * N + i];
+                                const float4 pj = s_pos[j];
+                                const float4 rj = s_req[j];
+                                const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                                float fr, ep;
+                                lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                                fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
+                                f -= d * fr;
+                                float4 a = facc[j];
+                                a.x += d.x * fr;
+                                a.y += d.y * fr;
+                                a.z += d.z * fr;
+                                facc[j] = a;
+                            }
+                        }
+                        // weighted corrections (ring topologies; empty for trees)
+                        for (int c = 0; c < I.ncorr; c++) {
+                            const PairCorrDev pc = B.corr[I.corr0 + c];
+                            if (pc.i != i) continue;
+                            const float4 pj = s_pos[pc.j];
+                            const float4 rj = s_req[pc.j];
+                            const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                            float fr, ep;
+                            lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                            f -= d * (fr * pc.w);
+                        }
                     }
-                    double ec = 0.0;  // directed corrections carry half of the pair energy each
-                    for (int c = 0; c < I.ncorr; c++) {
-                        const PairCorrDev pc = B.corr[I.corr0 + c];
-                        if (pc.i != i) continue;
-                        const float4 pj = s_pos[pc.j];
-                        const float4 rj = s_req[pc.j];
-                        float fr, ep;
-                        lj_pair<true, true>(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), ri.x + rj.x, ri.y * rj.y,
-                                            kQi * rj.z, ri.w * rj.w, fr, ep);
-                        ec += (double)ep * pc.w;
+                    if (P.bGridFF) {
+                        float4* gc = B.gridcache + (size_t)(I.a0 + i) * 16;
+                        grid_sample_cached(P.grid, pi, mk(plq.x, plq.y, plq.z), gc, gkey, f);
                     }
-                    epart += e + 0.5 * ec;
-                }
-                if (hasAtom && P.bGridFF) {
-                    V3 fd = zero3();
-                    epart += (double)grid_sample(P.grid, xyz(s_pos[tid]), mk(plq.x, plq.y, plq.z), fd);
+                    s_fnb[i] = make_float4(f.x, f.y, f.z, 0.f);
                 }
                 if (hasNode) {
                     V3 fbs[4], fps[4], fa, fpi;
                     auto pos = [&](int v) { return xyz(s_pos[v]); };
-                    epart += (double)eval_node(tid, N, M, pos, ng, apar, bL, bK, ksp, kpp, P.sw, fbs, fps, fa, fpi);
+                    eval_node(tid, N, M, pos, ng, apar, bL, bK, ksp, kpp, P.sw, fbs, fps, fa, fpi);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        s_fng[tid * 4 + k] = make_float4(fbs[k].x, fbs[k].y, fbs[k].z, 0.f);
+                        s_fng[4 * M + tid * 4 + k] = make_float4(fps[k].x, fps[k].y, fps[k].z, 0.f);
+                    }
+                    s_fself[tid] = make_float4(fa.x, fa.y, fa.z, 0.f);
+                    s_fself[M + tid] = make_float4(fpi.x, fpi.y, fpi.z, 0.f);
                 }
-                double ed = wave_sum_d(epart);
-                if (lane == 0) s_redd[wid] = ed;
                 __syncthreads();
-                if (tid == 0) {
-                    double E = 0.0;
-                    for (int w = 0; w < NW; w++) E += s_redd[w];
-                    B.Esys[s] = E;
-                    B.nsteps[s] = step;
-                    B.status[s] = status;
-                    B.Fmax[s] = sqrtf(fm2);
-                }
-                break;
-            }
 
-            // ================= C: FIRE + move
-            float cv, cfe;
-            fire_update(P.fire, fs, vf, vv, ff, cv, cfe);
-            const float dt = fs.dt;
+                // ================= B: assemble + reductions
+                float vf = 0.f, vv = 0.f, ff = 0.f, fm2 = 0.f;
+#pragma unroll
+                for (int r = 0; r < 2; r++) {
+                    const int v = tid + r * BLOCK;
+                    fv[r] = zero3();
+                    if (v < V) {
+                        V3 f = zero3();
+                        const bool isPi = v >= N;
+                        if (!isPi) {
+                            f = xyz(s_fnb[v]);
+                            for (int w = 0; w < nwAct; w++) f += xyz(s_facc[w * L.nmax + v]);
+                            if (v < M) f += xyz(s_fself[v]);
+                        } else {
+                            f = xyz(s_fself[M + (v - N)]);
+                        }
+                        const int b4[4] = {bk[r].x, bk[r].y, bk[r].z, bk[r].w};
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            if (b4[k] < 0) break;
+                            f += xyz(s_fng[b4[k]]);
+                        }
+                        if (isPi) {
+                            const V3 p = xyz(s_pos[v]);
+                            f -= p * dot(p, f);
+                        }
+                        fm2 = fmaxf(fm2, dot(f, f));  // convergence always on the exact force
+                        fv[r] = f;
+                    }
+                }
+                if (PC == 2 && inkernel && !fresh && step % P.pc_every == 0) {  // uniform across the block
+                    if (hasAtom) s_fnb[tid] = make_float4(fv[0].x, fv[0].y, fv[0].z, 0.f);
+                    propose_now = true;
+                    break;
+                }
+                if (PC)
+                    precondition<BLOCK>(P, s_pos, s_fnb, s_fself, s_fng, &s_cred[0][0][0], &s_csum[0][0], s_gain,
+                                        learned, N, M, nc, myc, hasAtom, hasNode, ng, bk[0], fv[0], [] {});
+#pragma unroll
+                for (int r = 0; r < 2; r++) {
+                    const int v = tid + r * BLOCK;
+                    if (v < V) {
+                        V3 f = fv[r];
+                        const float f2 = dot(f, f);
+                        if (P.Flimit > 0.f && f2 > P.Flimit * P.Flimit) f = f * (P.Flimit / sqrtf(f2));
+                        fv[r] = f;
+                        vf += dot(vel[r], f);
+                        vv += dot(vel[r], vel[r]);
+                        ff += dot(f, f);
+                    }
+                }
+                vf = wave_sum(vf);
+                vv = wave_sum(vv);
+                ff = wave_sum(ff);
+                fm2 = wave_max(fm2);
+                if (lane == 0) {
+                    s_red[0][wid] = vf;
+                    s_red[1][wid] = vv;
+                    s_red[2][wid] = ff;
+                    s_red[3][wid] = fm2;
+                }
+                __syncthreads();
+                vf = s_red[0][0];
+                vv = s_red[1][0];
+                ff = s_red[2][0];
+                fm2 = s_red[3][0];
+#pragma unroll
+                for (int w = 1; w < NW; w++) {
+                    vf += s_red[0][w];
+                    vv += s_red[1][w];
+                    ff += s_red[2][w];
+                    fm2 = fmaxf(fm2, s_red[3][w]);
+                }
+                fmax2_last = fm2;
+                const float crit = (P.conv_mode == CONV_F2SUM) ? ff : fm2;
+                if (!(ff == ff) || isinf(ff)) status = ST_NAN;
+                else if (crit < P.Fconv2) status = ST_CONVERGED;
+                else if (step >= P.nstep_max) status = ST_MAXSTEP;
+                if (status != ST_RUNNING) {
+                    // energy of this geometry: precise 1/r, FP64 accumulation and reduction
+                    double epart = 0.0;
+                    if (hasAtom && P.bNonBonded) {
+                        const int i = tid;
+                        const V3 pi = xyz(s_pos[i]);
+                        const float4 ri = s_req[i];
+                        const float kQi = COULOMB_K * ri.z;
+                        double e = 0.0;  // each unordered pair once (half ring)
+                        const int Kh = N >> 1;
+                        for (int k = 1; k <= Kh; k++) {
+                            if (!(N & 1) && k == Kh && i >= Kh) break;
+                            if ((s_mask[(k >> 5) * N + i] >> (k & 31)) & 1u) continue;
+                            int j = i + k;
+                            j = (j >= N) ? j - N : j;
+                            const float4 pj = s_pos[j];
+                            const float4 rj = s_req[j];
+                            float fr, ep;
+                            lj_pair<true, true>(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), ri.x + rj.x, ri.y * rj.y,
+                                                kQi * rj.z, ri.w * rj.w, fr, ep);
+                            e += (double)ep;
+                        }
+                        double ec = 0.0;  // directed corrections carry half of the pair energy each
+                        for (int c = 0; c < I.ncorr; c++) {
+                            const PairCorrDev pc = B.corr[I.corr0 + c];
+                            if (pc.i != i) continue;
+                            const float4 pj = s_pos[pc.j];
+                            const float4 rj = s_req[pc.j];
+                            float fr, ep;
+                            lj_pair<true, true>(mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z), ri.x + rj.x, ri.y * rj.y,
+                                                kQi * rj.z, ri.w * rj.w, fr, ep);
+                            ec += (double)ep * pc.w;
+                        }
+                        epart += e + 0.5 * ec;
+                    }
+                    if (hasAtom && P.bGridFF) {
+                        V3 fd = zero3();
+                        epart += (double)grid_sample(P.grid, xyz(s_pos[tid]), mk(plq.x, plq.y, plq.z), fd);
+                    }
+                    if (hasNode) {
+                        V3 fbs[4], fps[4], fa, fpi;
+                        auto pos = [&](int v) { return xyz(s_pos[v]); };
+                        epart += (double)eval_node(tid, N, M, pos, ng, apar, bL, bK, ksp, kpp, P.sw, fbs, fps, fa, fpi);
+                    }
+                    double ed = wave_sum_d(epart);
+                    if (lane == 0) s_redd[wid] = ed;
+                    __syncthreads();
+                    if (tid == 0) {
+                        double E = 0.0;
+                        for (int w = 0; w < NW; w++) E += s_redd[w];
+                        B.Esys[s] = E;
+                        B.nsteps[s] = step;
+                        B.status[s] = status;
+                        B.Fmax[s] = sqrtf(fm2);
+                    }
+                    break;
+                }
+
+                // ================= C: FIRE + move
+                float cv, cfe;
+                fire_update(P.fire, fs, vf, vv, ff, cv, cfe);
+                const float dt = fs.dt;
+#pragma unroll
+                for (int r = 0; r < 2; r++) {
+                    const int v = tid + r * BLOCK;
+                    if (v < V) {
+                        const bool isPi = v >= N;
+                        V3 p = xyz(s_pos[v]);
+                        V3 f = fv[r];
+                        if (isPi) f -= p * dot(p, f);
+                        V3 w = vel[r] * cv + f * (dt + cfe);
+                        if (isPi) w -= p * dot(p, w);
+                        p += w * dt;
+                        if (isPi) p = p * (1.f / sqrtf(dot(p, p)));
+                        vel[r] = w;
+                        s_pos[v] = make_float4(p.x, p.y, p.z, 0.f);
+                    }
+                }
+                step++;
+                fresh = false;
+                __syncthreads();
+            }
+            if (!propose_now) break;
+            // ---- network evaluation between segments (learned form)
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const int v = tid + r * BLOCK;
-                if (v < V) {
-                    const bool isPi = v >= N;
-                    V3 p = xyz(s_pos[v]);
-                    V3 f = fv[r];
-                    if (isPi) f -= p * dot(p, f);
-                    V3 w = vel[r] * cv + f * (dt + cfe);
-                    if (isPi) w -= p * dot(p, w);
-                    p += w * dt;
-                    if (isPi) p = p * (1.f / sqrtf(dot(p, p)));
-                    vel[r] = w;
-                    s_pos[v] = make_float4(p.x, p.y, p.z, 0.f);
-                }
+                if (v < V) B.avel[I.v0 + v] = make_float4(vel[r].x, vel[r].y, vel[r].z, 0.f);
             }
-            step++;
-            fresh = false;
             __syncthreads();
-        }
-        if (!propose_now) break;
-        // ---- network evaluation between segments (learned form)
+            propose_gains<BLOCK>(P, B.neighs + I.a0, B.REQs + I.a0, B.elem + I.a0, s_pos, s_fnb, s_req, s_h, g_h,
+                                 s_gain, &s_cred[0][0][0], N, M, nc, hasAtom ? B.comp[I.a0 + tid] : 0,
+                                 hasAtom ? B.PLQs[I.a0 + tid] : make_float4(0, 0, 0, 0));
+            if (L.h_aliased()) {  // the exclusion rows were overwritten by the activations
+                for (int w = tid; w < N * W; w += BLOCK) s_mask[(w % W) * N + (w / W)] = B.excl_rot[I.mask0 + w];
+            }
 #pragma unroll
-        for (int r = 0; r < 2; r++) {
-            const int v = tid + r * BLOCK;
-            if (v < V) B.avel[I.v0 + v] = make_float4(vel[r].x, vel[r].y, vel[r].z, 0.f);
-        }
-        __syncthreads();
-        propose_gains<BLOCK>(P, B.neighs + I.a0, B.REQs + I.a0, B.elem + I.a0, s_pos, s_fnb, s_req, s_h, g_h,
-                             s_gain, &s_cred[0][0][0], N, M, nc, hasAtom ? B.comp[I.a0 + tid] : 0,
-                             hasAtom ? B.PLQs[I.a0 + tid] : make_float4(0, 0, 0, 0));
-        if (L.h_aliased()) {  // the exclusion rows were overwritten by the activations
-            for (int w = tid; w < N * W; w += BLOCK) s_mask[(w % W) * N + (w / W)] = B.excl_rot[I.mask0 + w];
-        }
-#pragma unroll
-        for (int r = 0; r < 2; r++) {
-            const int v = tid + r * BLOCK;
-            if (v < V) vel[r] = xyz(B.avel[I.v0 + v]);
-        }
-        fresh = true;
-        __syncthreads();
+            for (int r = 0; r < 2; r++) {
+                const int v = tid + r * BLOCK;
+                if (v < V) vel[r] = xyz(B.avel[I.v0 + v]);
+            }
+            fresh = true;
+            __syncthreads();
         }
         (void)fmax2_last;
         if (suspended && tid == 0) {
