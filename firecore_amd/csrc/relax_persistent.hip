@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
     const bool learned = PC >= 2 && L.learned && P.pc_mode == 2;
     const bool inkernel = learned && P.chunk == 0;  // network evaluated inside this kernel
     constexpr int NW = BLOCK / 64;
-    __shared__ float s_red[4][NW];
+    __shared__ float s_red[4][2 * NW];
     __shared__ double s_redd[NW];
     __shared__ int s_q;
     // preconditioner: per-wave partial sums and per-component totals (PC_NSUM values each)
@@ -356,41 +356,40 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                 if (PC)
                     precondition<BLOCK>(P, s_pos, s_fnb, s_fself, s_fng, &s_cred[0][0][0], &s_csum[0][0], s_gain,
                                         learned, N, M, nc, myc, hasAtom, hasNode, ng, bk[0], fv[0], [] {});
+                // FIRE sums: totals per chunk of 64 consecutive vecs (vec v = tid + r*BLOCK
+                // lies in chunk wid + r*NW), added in chunk order, so that the result does
+                // not depend on the workgroup width a system is relaxed with
 #pragma unroll
                 for (int r = 0; r < 2; r++) {
                     const int v = tid + r * BLOCK;
+                    float pvf = 0.f, pvv = 0.f, pff = 0.f;
                     if (v < V) {
                         V3 f = fv[r];
                         const float f2 = dot(f, f);
                         if (P.Flimit > 0.f && f2 > P.Flimit * P.Flimit) f = f * (P.Flimit / sqrtf(f2));
                         fv[r] = f;
-                        vf += dot(vel[r], f);
-                        vv += dot(vel[r], vel[r]);
-                        ff += dot(f, f);
+                        pvf = dot(vel[r], f);
+                        pvv = dot(vel[r], vel[r]);
+                        pff = dot(f, f);
                     }
+                    const float t = wave_sum4(pvf, pvv, pff, 0.f);  // lane l: total of value l >> 4
+                    if ((lane & 15) == 0 && lane < 48) s_red[lane >> 4][wid + r * NW] = t;
                 }
-                vf = wave_sum(vf);
-                vv = wave_sum(vv);
-                ff = wave_sum(ff);
                 fm2 = wave_max(fm2);
-                if (lane == 0) {
-                    s_red[0][wid] = vf;
-                    s_red[1][wid] = vv;
-                    s_red[2][wid] = ff;
-                    s_red[3][wid] = fm2;
-                }
+                if (lane == 0) s_red[3][wid] = fm2;
                 __syncthreads();
-                vf = s_red[0][0];
-                vv = s_red[1][0];
-                ff = s_red[2][0];
+                vf = 0.f;
+                vv = 0.f;
+                ff = 0.f;
+#pragma unroll
+                for (int c = 0; c < 2 * NW; c++) {
+                    vf += s_red[0][c];
+                    vv += s_red[1][c];
+                    ff += s_red[2][c];
+                }
                 fm2 = s_red[3][0];
 #pragma unroll
-                for (int w = 1; w < NW; w++) {
-                    vf += s_red[0][w];
-                    vv += s_red[1][w];
-                    ff += s_red[2][w];
-                    fm2 = fmaxf(fm2, s_red[3][w]);
-                }
+                for (int w = 1; w < NW; w++) fm2 = fmaxf(fm2, s_red[3][w]);
                 fmax2_last = fm2;
                 const float crit = (P.conv_mode == CONV_F2SUM) ? ff : fm2;
                 if (!(ff == ff) || isinf(ff)) status = ST_NAN;

@@ -232,7 +232,9 @@ int get_last_chunks(void);
 int set_graphs(int on);
 /* Persistent-path scheduling (no counterpart in the reference): 0 = slots
  * split between size classes by estimated time with work stealing, 1 = every
- * class at full device width, no stealing.  Returns the previous mode. */
+ * class at full device width, no stealing, 2 = every class at twice its
+ * resident width (the extra workgroups start as other classes retire).
+ * Returns the previous mode, or -1 for an unknown mode. */
 int set_sched(int mode);
 /* nSys, ntotA, ntotM, ntotV, maxN, maxM, grid nx, ny, nz */
 void get_batch_sizes(int* out9);

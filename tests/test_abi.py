@@ -42,3 +42,19 @@ def test_reference_bound_symbols_exported():
 def test_python_module_mirror_imports():
     from firecore_amd import MMFF_multi  # noqa: F401  (binds every symbol at import, as the reference module)
     assert MMFF_multi.lib is native.lib()
+
+
+def test_host_setters_validate_their_arguments():
+    """Mode setters check their argument without touching a device."""
+    old = native.set_sched(2)
+    try:
+        assert native.set_sched(1) == 2
+        for bad in (-1, 3):
+            try:
+                native.set_sched(bad)
+            except ValueError:
+                pass
+            else:
+                raise AssertionError("set_sched accepted %d" % bad)
+    finally:
+        native.set_sched(old)

@@ -187,6 +187,44 @@ def test_persistent_matches_split(nat):
     nat.set_force_limit(0.0)
 
 
+@pytest.mark.parametrize("form", ["plain", "learned"])
+def test_schedules_give_identical_results(nat, form):
+    """One workgroup relaxes a system from its first step to its last with the
+    same code whatever the scheduling: every scheduler mode, and the autotuned
+    queue order of a repeated pass, give bit-identical per-system results."""
+    import json
+    import os
+    P("case: schedulers", form)
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "firecore_amd")
+    nat.set_opt(*FIRE_MULTI)
+    nat.set_force_limit(10.0)
+    nat.init_workload(1, 96, 30001, 16, 256, 0.1, "synth:NaCl", 0.1)
+    if form == "learned":
+        meta = json.load(open(os.path.join(here, "proposal_weights.json")))
+        nat.set_proposal_weights(np.load(os.path.join(here, "proposal_weights.npy")))
+        nat.set_proposal_every(16)
+        nat.set_precond(2, meta["gstar"], meta["grigid"])
+    old = nat.set_sched(1)
+    runs = []
+    try:
+        for mode in (1, 1, 0, 2):
+            nat.set_sched(mode)
+            nat.relax(10000, 1e-3, 0, True, True)
+            B = nat.Batch()
+            pos = np.concatenate([B.positions(s) for s in range(B.nsys)])
+            runs.append((mode, np.array(B.status), np.array(B.nsteps), np.array(B.E), pos))
+    finally:
+        nat.set_sched(old)
+        nat.set_precond(0)
+        nat.set_force_limit(0.0)
+    assert len(set(int(c["block"]) for c in nat.class_info())) >= 3
+    _, st0, ns0, E0, p0 = runs[0]
+    assert (st0 == nat.ST_CONVERGED).sum() >= 80, np.bincount(st0, minlength=4)
+    for mode, st, ns, E, p in runs[1:]:
+        assert (st == st0).all() and (ns == ns0).all(), mode
+        assert np.array_equal(E, E0) and np.array_equal(p, p0), mode
+
+
 def test_nonfinite_system_is_isolated(nat):
     """A system with two coincident atoms goes non-finite at step 0 and is
     retired as failed without affecting the other systems."""
