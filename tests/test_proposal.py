@@ -263,6 +263,43 @@ def test_learned_relaxation_converges_to_oracle_minima(nacl_grid, orc, inkernel)
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("inkernel", [2, 0])
+def test_zero_network_reproduces_constant_gains(inkernel):
+    """With every weight and bias zero the network outputs exactly 0, so its
+    gains are exactly gstar/2 and grigid/2 (sigma(0) = 1/2): the learned form
+    must then reproduce the constant-gain form bit for bit.  With a short
+    period this checks every part of the proposal machinery (segment breaks and
+    restarts with re-evaluated forces, parked velocities, restaged exclusion
+    and REQ rows; chunk suspension and resumption) against a path that has none
+    of it, over all four size classes."""
+    native.set_opt(0.1, 0.02, 0.2, 1.1, 0.5, 0.8, 5)
+    native.set_force_limit(10.0)
+    native.init_workload(1, 48, 9700, 16, 256, 0.1, "synth:NaCl", 0.1)
+    native.set_proposal_weights(np.zeros(native.proposal_nparam(), np.float32))
+    old = native.set_learned_inkernel(inkernel)
+    runs = []
+    try:
+        for mode, gs, gr in [(1, 2.0, 63.0), (2, 4.0, 126.0)]:
+            native.set_precond(mode, gs, gr)
+            native.set_proposal_every(8)
+            native.relax(10000, 1e-3, 0, True, True)
+            B = native.Batch()
+            pos = np.concatenate([B.positions(s) for s in range(B.nsys)])
+            runs.append((np.array(B.status), np.array(B.nsteps), np.array(B.E), pos))
+        assert len(native.class_info()) == 4
+        assert (native.last_chunks() > 0) == (inkernel == 0)
+    finally:
+        native.set_learned_inkernel(old)
+        native.set_precond(0)
+        native.set_force_limit(0.0)
+        native.set_opt()
+    (st0, ns0, E0, p0), (st1, ns1, E1, p1) = runs
+    assert (st0 == native.ST_CONVERGED).all()
+    assert (st1 == st0).all() and (ns1 == ns0).all()
+    assert np.array_equal(E1, E0) and np.array_equal(p1, p0)
+
+
+@pytest.mark.gpu
 def test_training_lowers_the_loss():
     _setup_batch(64, 9600, 16, 128, theta_seed=11)
     try:
