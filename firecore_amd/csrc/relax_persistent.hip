@@ -34,15 +34,16 @@ struct BlockRed {
 //   pos   float4[nmax+mmax]     positions of atoms and pi-vectors
 //   req   float4[nmax]          (R0, sqrt(E0), Q, H)
 //   fnb   float4[nmax]          non-bonded + grid force accumulated by atom i's thread
-//   facc  float4[nwave*nmax]    per-wave reaction forces of the half-ring schedule
+//   facc  float4[nwave*frow]    per-wave reaction forces of the half-ring schedule:
+//                              wave w's row holds partners 64w+1 .. 64w+63+nmax/2
 //   fself float4[2*mmax]        bonded self forces of nodes and pi-vectors
 //   fng   float4[8*mmax]        recoils (sigma rows, then pi rows)
-//   mask  uint[wmax*nmax]       rotated exclusion rows, [word][atom]
+//   mask  uint[wmax*nmax]       rotated exclusion rows, [word][atom]; only the
+//                              words of offsets k <= nmax/2 (wmax = nmax/64 + 1)
 //   learned gains only:
 //   gain  float[mmax+PC_MAXC]   gains of the stars and the components
-//   out   float[2*nmax]         network outputs
-//   h     bf16 rows            network activations (one of the two buffers;
-//                              aliased onto facc..fng when it fits there)
+//   h     bf16 rows            network activations (one of the two buffers),
+//                              aliased onto facc..mask, which is padded to fit
 struct LdsLayout {
     int nmax, mmax, wmax, nwave, learned, h2lds;
     __host__ __device__ static int r16(int b) { return (b + 15) & ~15; }
@@ -51,17 +52,22 @@ struct LdsLayout {
     __host__ __device__ int off_req() const { return off_pos() + vmax() * 16; }
     __host__ __device__ int off_fnb() const { return off_req() + nmax * 16; }
     __host__ __device__ int off_facc() const { return off_fnb() + nmax * 16; }
-    __host__ __device__ int off_fself() const { return off_facc() + nwave * nmax * 16; }
+    // entries of a wave's reaction row: lane l's partner at offset k lands at l+k-1
+    __host__ __device__ int frow() const { return 64 + (nmax >> 1); }
+    __host__ __device__ int off_fself() const { return off_facc() + nwave * frow() * 16; }
     __host__ __device__ int off_fng() const { return off_fself() + 2 * mmax * 16; }
     __host__ __device__ int off_mask() const { return off_fng() + 8 * mmax * 16; }
     __host__ __device__ int end_base() const { return off_mask() + nmax * wmax * 4; }
-    __host__ __device__ int off_gain() const { return r16(end_base()); }
-    __host__ __device__ int off_hx() const { return off_gain() + r16((mmax + PC_MAXC) * 4); }
     __host__ __device__ int hbytes() const { return r16(nmax * MLP_HS * 2); }
-    // first activation buffer: in LDS when it fits over facc..mask (free during
-    // the network evaluation; the exclusion rows are staged again after it),
-    // else in the workgroup's global slot
-    __host__ __device__ bool h_aliased() const { return hbytes() <= end_base() - off_facc(); }
+    __host__ __device__ int off_gain() const {
+        const int e = end_base();
+        return r16((learned && off_facc() + hbytes() > e) ? off_facc() + hbytes() : e);
+    }
+    __host__ __device__ int off_hx() const { return off_gain() + r16((mmax + PC_MAXC) * 4); }
+    // first activation buffer: in LDS over facc..mask (free during the network
+    // evaluation; the exclusion rows are staged again after it), else in the
+    // workgroup's global slot
+    __host__ __device__ bool h_aliased() const { return hbytes() <= off_gain() - off_facc(); }
     __host__ __device__ int off_h2() const { return off_hx(); }
     __host__ __device__ int bytes() const {
         if (!learned) return end_base();
@@ -137,13 +143,20 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
         const int s = q;
         const SysInfo I = B.info[s];
         const int N = I.natoms, M = I.nnode, V = N + M;
-        const int W = (N + 31) >> 5;
+        const int W = (N + 31) >> 5;         // exclusion words per atom in memory
+        const int Wh = ((N >> 1) >> 5) + 1;  // words of the offsets k <= N/2 the half ring reads
+        // exclusion rows transposed to [word][atom] so that lanes read consecutive banks
+        auto stage_mask = [&] {
+            for (int w = tid; w < N * Wh; w += BLOCK) {
+                const int a = w / Wh, wd = w - a * Wh;
+                s_mask[wd * N + a] = B.excl_rot[I.mask0 + a * W + wd];
+            }
+        };
 
         // ---- stage the system in LDS
         for (int v = tid; v < V; v += BLOCK) s_pos[v] = B.atoms[I.v0 + v];
         for (int i = tid; i < N; i += BLOCK) s_req[i] = B.REQs[I.a0 + i];
-        // transposed to [word][atom] so that lanes read consecutive banks
-        for (int w = tid; w < N * W; w += BLOCK) s_mask[(w % W) * N + (w / W)] = B.excl_rot[I.mask0 + w];
+        stage_mask();
 
         const bool hasAtom = tid < N;
         const bool hasNode = tid < M;
@@ -229,12 +242,17 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                 // Half-ring pair schedule (Newton's third law): thread i takes the
                 // partners j = (i+k) mod N, k = 1..N/2, so every unordered pair is
                 // computed once.  The reaction -f_ij goes to the wave's private
-                // accumulator row: within one instruction the lanes of a wave hold
-                // distinct j, and no other wave writes that row.
+                // accumulator row at entry lane+k-1 (the partners of a wave form
+                // a window that slides with k): within one instruction the lanes
+                // of a wave hold distinct entries, and no other wave writes that row.
                 const int nwAct = (N + 63) >> 6;
-                float4* facc = s_facc + wid * L.nmax;
+                const int nrow = 64 + (N >> 1);  // entries a wave's row may receive
+                float4* facc = s_facc + wid * L.frow();
                 if (wid < nwAct)
-                    for (int j = lane; j < N; j += 64) facc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    for (int j = lane; j < nrow; j += 64) facc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                // the row is shared by the lanes of the wave: keep every LDS access to it
+                // in program order (the compiler only sees each lane's own addresses)
+                __builtin_amdgcn_wave_barrier();
                 if (hasAtom) {
                     const int i = tid;
                     const V3 pi = xyz(s_pos[i]);
@@ -260,11 +278,12 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                                 f.x -= d.x * fr;
                                 f.y -= d.y * fr;
                                 f.z -= d.z * fr;
-                                float4 a = facc[j];
+                                float4 a = facc[lane + k - 1];
                                 a.x += d.x * fr;
                                 a.y += d.y * fr;
                                 a.z += d.z * fr;
-                                facc[j] = a;
+                                facc[lane + k - 1] = a;
+                                __builtin_amdgcn_wave_barrier();
                             }
                         }
                         if (!(N & 1)) {  // k = N/2 pairs (i, i+N/2): taken by the lower half only
@@ -279,11 +298,12 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                                 lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
                                 fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
                                 f -= d * fr;
-                                float4 a = facc[j];
+                                float4 a = facc[lane + k - 1];
                                 a.x += d.x * fr;
                                 a.y += d.y * fr;
                                 a.z += d.z * fr;
-                                facc[j] = a;
+                                facc[lane + k - 1] = a;
+                                __builtin_amdgcn_wave_barrier();
                             }
                         }
                         // weighted corrections (ring topologies; empty for trees)
@@ -329,7 +349,13 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                         const bool isPi = v >= N;
                         if (!isPi) {
                             f = xyz(s_fnb[v]);
-                            for (int w = 0; w < nwAct; w++) f += xyz(s_facc[w * L.nmax + v]);
+                            for (int w = 0; w < nwAct; w++) {  // atom v sits at entry (v-64w-1) mod N (+N)
+                                int e = v - 64 * w - 1;
+                                e = (e < 0) ? e + N : e;
+                                const float4* row = s_facc + w * L.frow();
+                                if (e < nrow) f += xyz(row[e]);
+                                if (e + N < nrow) f += xyz(row[e + N]);
+                            }
                             if (v < M) f += xyz(s_fself[v]);
                         } else {
                             f = xyz(s_fself[M + (v - N)]);
@@ -489,7 +515,7 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                                  s_gain, &s_cred[0][0][0], N, M, nc, hasAtom ? B.comp[I.a0 + tid] : 0,
                                  hasAtom ? B.PLQs[I.a0 + tid] : make_float4(0, 0, 0, 0));
             if (L.h_aliased()) {  // the exclusion rows were overwritten by the activations
-                for (int w = tid; w < N * W; w += BLOCK) s_mask[(w % W) * N + (w / W)] = B.excl_rot[I.mask0 + w];
+                stage_mask();
             }
 #pragma unroll
             for (int r = 0; r < 2; r++) {

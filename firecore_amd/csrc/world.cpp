@@ -494,7 +494,7 @@ void World::plan_classes() {
         mrun = std::max(mrun, C.mmax);
         C.nmax = nrun;
         C.mmax = mrun;
-        C.wmax = (C.nmax + 31) / 32;
+        C.wmax = ((C.nmax >> 1) >> 5) + 1;  // exclusion words of the offsets k <= nmax/2
         // learned gains: the second activation buffer goes to LDS when the class
         // keeps two resident workgroups per CU pair of SIMDs (see plan below)
         C.h2lds = 0;
