@@ -10,15 +10,19 @@
 //
 // Per FIRE step (reference recipe MolWorld_sp3::run_no_omp,
 // MolWorld_sp3.h:2291-2474):
-//   A. forces: thread i -> non-bonded sum of atom i over all j of its system
-//      (positions/params broadcast from LDS, exclusions from an LDS bit
-//      matrix) + B-spline GridFF sample; thread k -> bonded terms of node k
-//      (recoils into LDS, MMFFsp3_loc.h:250-575)
-//   B. assemble: thread v -> own force + recoils via back-neighbours, pi force
-//      projected perpendicular to the pi-vector (assemble_atom, :998-1021);
-//      block reduction of <v|f>, <v|v>, <f|f> and max|f|^2
+//   A. forces: thread i -> non-bonded pairs (i, i+k mod N), k <= N/2, with the
+//      reactions in per-wave LDS rows (positions/params from LDS, exclusions
+//      from an LDS bit matrix) + B-spline GridFF sample through the atom's
+//      window cache; thread k -> bonded terms of node k (recoils into LDS,
+//      MMFFsp3_loc.h:250-575)
+//   B. assemble: thread v -> own force + reactions + recoils via
+//      back-neighbours, pi force projected perpendicular to the pi-vector
+//      (assemble_atom, :998-1021); optional rigid-body preconditioner
+//      (precond.h); block reduction of <v|f>, <v|v>, <f|f> and max|f|^2
 //   C. per-system FIRE (DynamicOpt.cpp:250-267) and move (move_atom_FIRE,
 //      MMFFsp3_loc.h:1264-1281)
+// The learned form pauses the step loop every P steps to evaluate the
+// proposal network (proposal.h) and restarts that step with the new gains.
 #include "kernels.h"
 #include "precond.h"
 #include "proposal.h"
