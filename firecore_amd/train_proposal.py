@@ -1,6 +1,6 @@
 """Train the proposal network on the GPU and write its parameters.
 
-    python -m firecore_amd.train_proposal [--systems 4096] [--iters 600] [--out PATH]
+    python -m firecore_amd.train_proposal [--systems 4096] [--iters 4000] [--out PATH]
 
 Training data: plain-FIRE relaxations (the library's persistent kernel, no
 preconditioner) of synthetic molecules N ~ U[16,256] placed over the NaCl(001)
@@ -27,7 +27,7 @@ from firecore_amd import native  # noqa: E402
 
 WEIGHTS = os.path.join(HERE, "proposal_weights.npy")
 FIRE = (0.1, 0.02, 0.2, 1.1, 0.5, 0.8, 5)
-GSTAR, GRIGID = 3.0, 127.0        # gain ceilings of the network's outputs
+GSTAR, GRIGID = 3.0, 127.0        # default gain ceilings of the network's outputs
 G0_STAR, G0_RIGID = 2.0, 63.0     # initial gains (the constant-gain setting)
 TRAIN_SEED0 = 1000001
 
@@ -59,10 +59,10 @@ def main():
     ap.add_argument("--eta", type=float, default=0.02)
     ap.add_argument("--dmax", type=float, default=0.5)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--gstar", type=float, default=2.5)
+    ap.add_argument("--gstar", type=float, default=GSTAR)
     ap.add_argument("--grigid", type=float, default=GRIGID)
     ap.add_argument("--out", default=WEIGHTS)
-    ap.add_argument("--every-candidates", type=int, nargs="+", default=[8, 16, 32])
+    ap.add_argument("--every-candidates", type=int, nargs="+", default=[32, 64, 128, 256])
     args = ap.parse_args()
 
     t0 = time.time()
@@ -94,13 +94,14 @@ def main():
               flush=True)
     theta = native.get_proposal_weights()
     native.record_snapshots(1, 0)
-    # proposal period: the fastest of the candidates on training systems (the
-    # first --tune-systems of this batch), measured with the trained weights
+    # proposal period: the fastest of the candidates on the training systems,
+    # measured with the trained weights (second pass, longest-first order)
     native.set_proposal_weights(theta)
     native.set_precond(2, args.gstar, args.grigid)
     best = None
     for every in args.every_candidates:
         native.set_proposal_every(every)
+        native.relax(10000, 1e-3, 0, True, False)  # step counts for the longest-first queue order
         native.relax(10000, 1e-3, 0, True, False)
         ms = native.last_ms()
         native.lib().download(True, True)
