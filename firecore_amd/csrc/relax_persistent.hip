@@ -199,10 +199,11 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
         V3 fv[2];
         // The learned form evaluates the network between segments of the step
         // loop (one segment per proposal period): the step loop breaks out at a
-        // proposal step, the network runs with the velocities parked in memory,
-        // and the segment restarts that step.  Nothing of the step loop is live
-        // across the network, so neither part adds to the other's registers.
-        bool fresh = false;  // gains of the current step already proposed
+        // proposal step once its exact forces are assembled, the network runs
+        // with the forces and velocities parked in memory, and the segment
+        // resumes that step at the preconditioner.  Nothing of the step loop is
+        // live across the network, so neither part adds to the other's registers.
+        bool fresh = false;  // gains of the current step proposed: resume it after its force assembly
         for (;;) {
             bool propose_now = false;
             // ---- per-thread state of this segment: atom tid, node tid, vecs tid and tid+BLOCK
@@ -242,144 +243,162 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                     float4* dst = B.snap + (size_t)(step / B.snap_every) * B.ntotV + I.v0;
                     for (int v = tid; v < V; v += BLOCK) dst[v] = s_pos[v];
                 }
-                // ================= A: forces (no energies inside the step loop)
-                // Half-ring pair schedule (Newton's third law): thread i takes the
-                // partners j = (i+k) mod N, k = 1..N/2, so every unordered pair is
-                // computed once.  The reaction -f_ij goes to the wave's private
-                // accumulator row at entry lane+k-1 (the partners of a wave form
-                // a window that slides with k): within one instruction the lanes
-                // of a wave hold distinct entries, and no other wave writes that row.
-                const int nwAct = (N + 63) >> 6;
-                const int nrow = 64 + (N >> 1);  // entries a wave's row may receive
-                float4* facc = s_facc + wid * L.frow();
-                if (wid < nwAct)
-                    for (int j = lane; j < nrow; j += 64) facc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-                // the row is shared by the lanes of the wave: keep every LDS access to it
-                // in program order (the compiler only sees each lane's own addresses)
-                __builtin_amdgcn_wave_barrier();
-                if (hasAtom) {
-                    const int i = tid;
-                    const V3 pi = xyz(s_pos[i]);
-                    const float4 ri = s_req[i];
-                    const float kQi = COULOMB_K * ri.z;
-                    V3 f = zero3();
-                    if (P.bNonBonded) {
-                        const int K = (N - 1) >> 1;
-                        for (int w = 0; (w << 5) <= K; w++) {
-                            const unsigned m = s_mask[w * N + i];
-                            const int k0 = max(1, w << 5);
-                            const int k1 = min(K, (w << 5) + 31);
-#pragma unroll 4
-                            for (int k = k0; k <= k1; k++) {
-                                int j = i + k;
-                                j = (j >= N) ? j - N : j;
-                                const float4 pj = s_pos[j];
-                                const float4 rj = s_req[j];
-                                const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                                float fr, ep;
-                                lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
-                                fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
-                                f.x -= d.x * fr;
-                                f.y -= d.y * fr;
-                                f.z -= d.z * fr;
-                                float4 a = facc[lane + k - 1];
-                                a.x += d.x * fr;
-                                a.y += d.y * fr;
-                                a.z += d.z * fr;
-                                facc[lane + k - 1] = a;
-                                __builtin_amdgcn_wave_barrier();
-                            }
-                        }
-                        if (!(N & 1)) {  // k = N/2 pairs (i, i+N/2): taken by the lower half only
-                            const int k = N >> 1;
-                            if (i < k) {
-                                const int j = i + k;
-                                const unsigned m = s_mask[(k >> 5) * N + i];
-                                const float4 pj = s_pos[j];
-                                const float4 rj = s_req[j];
-                                const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                                float fr, ep;
-                                lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
-                                fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
-                                f -= d * fr;
-                                float4 a = facc[lane + k - 1];
-                                a.x += d.x * fr;
-                                a.y += d.y * fr;
-                                a.z += d.z * fr;
-                                facc[lane + k - 1] = a;
-                                __builtin_amdgcn_wave_barrier();
-                            }
-                        }
-                        // weighted corrections (ring topologies; empty for trees)
-                        for (int c = 0; c < I.ncorr; c++) {
-                            const PairCorrDev pc = B.corr[I.corr0 + c];
-                            if (pc.i != i) continue;
-                            const float4 pj = s_pos[pc.j];
-                            const float4 rj = s_req[pc.j];
-                            const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                            float fr, ep;
-                            lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
-                            f -= d * (fr * pc.w);
-                        }
-                    }
-                    if (P.bGridFF) {
-                        float4* gc = B.gridcache + (size_t)(I.a0 + i) * 16;
-                        grid_sample_cached(P.grid, pi, mk(plq.x, plq.y, plq.z), gc, gkey, f);
-                    }
-                    s_fnb[i] = make_float4(f.x, f.y, f.z, 0.f);
-                }
-                if (hasNode) {
-                    V3 fbs[4], fps[4], fa, fpi;
-                    auto pos = [&](int v) { return xyz(s_pos[v]); };
-                    eval_node(tid, N, M, pos, ng, apar, bL, bK, ksp, kpp, P.sw, fbs, fps, fa, fpi);
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        s_fng[tid * 4 + k] = make_float4(fbs[k].x, fbs[k].y, fbs[k].z, 0.f);
-                        s_fng[4 * M + tid * 4 + k] = make_float4(fps[k].x, fps[k].y, fps[k].z, 0.f);
-                    }
-                    s_fself[tid] = make_float4(fa.x, fa.y, fa.z, 0.f);
-                    s_fself[M + tid] = make_float4(fpi.x, fpi.y, fpi.z, 0.f);
-                }
-                __syncthreads();
-
-                // ================= B: assemble + reductions
                 float vf = 0.f, vv = 0.f, ff = 0.f, fm2 = 0.f;
-#pragma unroll
-                for (int r = 0; r < 2; r++) {
-                    const int v = tid + r * BLOCK;
-                    fv[r] = zero3();
-                    if (v < V) {
+                if (!fresh) {
+                    // ================= A: forces (no energies inside the step loop)
+                    // Half-ring pair schedule (Newton's third law): thread i takes the
+                    // partners j = (i+k) mod N, k = 1..N/2, so every unordered pair is
+                    // computed once.  The reaction -f_ij goes to the wave's private
+                    // accumulator row at entry lane+k-1 (the partners of a wave form
+                    // a window that slides with k): within one instruction the lanes
+                    // of a wave hold distinct entries, and no other wave writes that row.
+                    const int nwAct = (N + 63) >> 6;
+                    const int nrow = 64 + (N >> 1);  // entries a wave's row may receive
+                    float4* facc = s_facc + wid * L.frow();
+                    if (wid < nwAct)
+                        for (int j = lane; j < nrow; j += 64) facc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    // the row is shared by the lanes of the wave: keep every LDS access to it
+                    // in program order (the compiler only sees each lane's own addresses)
+                    __builtin_amdgcn_wave_barrier();
+                    if (hasAtom) {
+                        const int i = tid;
+                        const V3 pi = xyz(s_pos[i]);
+                        const float4 ri = s_req[i];
+                        const float kQi = COULOMB_K * ri.z;
                         V3 f = zero3();
-                        const bool isPi = v >= N;
-                        if (!isPi) {
-                            f = xyz(s_fnb[v]);
-                            for (int w = 0; w < nwAct; w++) {  // atom v sits at entry (v-64w-1) mod N (+N)
-                                int e = v - 64 * w - 1;
-                                e = (e < 0) ? e + N : e;
-                                const float4* row = s_facc + w * L.frow();
-                                if (e < nrow) f += xyz(row[e]);
-                                if (e + N < nrow) f += xyz(row[e + N]);
+                        if (P.bNonBonded) {
+                            const int K = (N - 1) >> 1;
+                            for (int w = 0; (w << 5) <= K; w++) {
+                                const unsigned m = s_mask[w * N + i];
+                                const int k0 = max(1, w << 5);
+                                const int k1 = min(K, (w << 5) + 31);
+#pragma unroll 4
+                                for (int k = k0; k <= k1; k++) {
+                                    int j = i + k;
+                                    j = (j >= N) ? j - N : j;
+                                    const float4 pj = s_pos[j];
+                                    const float4 rj = s_req[j];
+                                    const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                                    float fr, ep;
+                                    lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                                    fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
+                                    f.x -= d.x * fr;
+                                    f.y -= d.y * fr;
+                                    f.z -= d.z * fr;
+                                    float4 a = facc[lane + k - 1];
+                                    a.x += d.x * fr;
+                                    a.y += d.y * fr;
+                                    a.z += d.z * fr;
+                                    facc[lane + k - 1] = a;
+                                    __builtin_amdgcn_wave_barrier();
+                                }
                             }
-                            if (v < M) f += xyz(s_fself[v]);
-                        } else {
-                            f = xyz(s_fself[M + (v - N)]);
+                            if (!(N & 1)) {  // k = N/2 pairs (i, i+N/2): taken by the lower half only
+                                const int k = N >> 1;
+                                if (i < k) {
+                                    const int j = i + k;
+                                    const unsigned m = s_mask[(k >> 5) * N + i];
+                                    const float4 pj = s_pos[j];
+                                    const float4 rj = s_req[j];
+                                    const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                                    float fr, ep;
+                                    lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                                    fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
+                                    f -= d * fr;
+                                    float4 a = facc[lane + k - 1];
+                                    a.x += d.x * fr;
+                                    a.y += d.y * fr;
+                                    a.z += d.z * fr;
+                                    facc[lane + k - 1] = a;
+                                    __builtin_amdgcn_wave_barrier();
+                                }
+                            }
+                            // weighted corrections (ring topologies; empty for trees)
+                            for (int c = 0; c < I.ncorr; c++) {
+                                const PairCorrDev pc = B.corr[I.corr0 + c];
+                                if (pc.i != i) continue;
+                                const float4 pj = s_pos[pc.j];
+                                const float4 rj = s_req[pc.j];
+                                const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                                float fr, ep;
+                                lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                                f -= d * (fr * pc.w);
+                            }
                         }
-                        const int b4[4] = {bk[r].x, bk[r].y, bk[r].z, bk[r].w};
+                        if (P.bGridFF) {
+                            float4* gc = B.gridcache + (size_t)(I.a0 + i) * 16;
+                            grid_sample_cached(P.grid, pi, mk(plq.x, plq.y, plq.z), gc, gkey, f);
+                        }
+                        s_fnb[i] = make_float4(f.x, f.y, f.z, 0.f);
+                    }
+                    if (hasNode) {
+                        V3 fbs[4], fps[4], fa, fpi;
+                        auto pos = [&](int v) { return xyz(s_pos[v]); };
+                        eval_node(tid, N, M, pos, ng, apar, bL, bK, ksp, kpp, P.sw, fbs, fps, fa, fpi);
 #pragma unroll
                         for (int k = 0; k < 4; k++) {
-                            if (b4[k] < 0) break;
-                            f += xyz(s_fng[b4[k]]);
+                            s_fng[tid * 4 + k] = make_float4(fbs[k].x, fbs[k].y, fbs[k].z, 0.f);
+                            s_fng[4 * M + tid * 4 + k] = make_float4(fps[k].x, fps[k].y, fps[k].z, 0.f);
                         }
-                        if (isPi) {
-                            const V3 p = xyz(s_pos[v]);
-                            f -= p * dot(p, f);
+                        s_fself[tid] = make_float4(fa.x, fa.y, fa.z, 0.f);
+                        s_fself[M + tid] = make_float4(fpi.x, fpi.y, fpi.z, 0.f);
+                    }
+                    __syncthreads();
+
+                    // ================= B: assemble + reductions
+#pragma unroll
+                    for (int r = 0; r < 2; r++) {
+                        const int v = tid + r * BLOCK;
+                        fv[r] = zero3();
+                        if (v < V) {
+                            V3 f = zero3();
+                            const bool isPi = v >= N;
+                            if (!isPi) {
+                                f = xyz(s_fnb[v]);
+                                for (int w = 0; w < nwAct; w++) {  // atom v sits at entry (v-64w-1) mod N (+N)
+                                    int e = v - 64 * w - 1;
+                                    e = (e < 0) ? e + N : e;
+                                    const float4* row = s_facc + w * L.frow();
+                                    if (e < nrow) f += xyz(row[e]);
+                                    if (e + N < nrow) f += xyz(row[e + N]);
+                                }
+                                if (v < M) f += xyz(s_fself[v]);
+                            } else {
+                                f = xyz(s_fself[M + (v - N)]);
+                            }
+                            const int b4[4] = {bk[r].x, bk[r].y, bk[r].z, bk[r].w};
+#pragma unroll
+                            for (int k = 0; k < 4; k++) {
+                                if (b4[k] < 0) break;
+                                f += xyz(s_fng[b4[k]]);
+                            }
+                            if (isPi) {
+                                const V3 p = xyz(s_pos[v]);
+                                f -= p * dot(p, f);
+                            }
+                            fm2 = fmaxf(fm2, dot(f, f));  // convergence always on the exact force
+                            fv[r] = f;
                         }
-                        fm2 = fmaxf(fm2, dot(f, f));  // convergence always on the exact force
-                        fv[r] = f;
+                    }
+                } else {
+                    // restart of a proposal step: its exact forces were parked before the network
+#pragma unroll
+                    for (int r = 0; r < 2; r++) {
+                        const int v = tid + r * BLOCK;
+                        fv[r] = zero3();
+                        if (v < V) {
+                            fv[r] = xyz(B.aforce[I.v0 + v]);
+                            fm2 = fmaxf(fm2, dot(fv[r], fv[r]));
+                        }
                     }
                 }
                 if (PC == 2 && inkernel && !fresh && step % P.pc_every == 0) {  // uniform across the block
                     if (hasAtom) s_fnb[tid] = make_float4(fv[0].x, fv[0].y, fv[0].z, 0.f);
+#pragma unroll
+                    for (int r = 0; r < 2; r++) {
+                        const int v = tid + r * BLOCK;
+                        if (v < V) B.aforce[I.v0 + v] = make_float4(fv[r].x, fv[r].y, fv[r].z, 0.f);
+                    }
                     propose_now = true;
                     break;
                 }
