@@ -68,14 +68,13 @@ struct LdsLayout {
         return r16((learned && off_facc() + hbytes() > e) ? off_facc() + hbytes() : e);
     }
     __host__ __device__ int off_hx() const { return off_gain() + r16((mmax + PC_MAXC) * 4); }
-    // first activation buffer: in LDS over facc..mask (free during the network
-    // evaluation; the exclusion rows are staged again after it), else in the
-    // workgroup's global slot
-    __host__ __device__ bool h_aliased() const { return hbytes() <= off_gain() - off_facc(); }
+    // first activation buffer: in LDS over facc..mask, which is free during the
+    // network evaluation (the exclusion rows are staged again after it) and is
+    // padded to hold it (off_gain)
     __host__ __device__ int off_h2() const { return off_hx(); }
     __host__ __device__ int bytes() const {
         if (!learned) return end_base();
-        return off_h2() + ((h2lds && h_aliased()) ? hbytes() : 0);
+        return off_h2() + (h2lds ? hbytes() : 0);
     }
 };
 
@@ -107,10 +106,9 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
     float* s_gain = reinterpret_cast<float*>(lds + L.off_gain());
     // activation buffers of the in-kernel network: LDS where the layout allows,
     // else the two halves of this workgroup's global slot
-    unsigned short* g_slot = Q.hslot ? Q.hslot + (size_t)blockIdx.x * L.hbytes() : nullptr;
-    unsigned short* s_h = L.h_aliased() ? reinterpret_cast<unsigned short*>(lds + L.off_facc()) : g_slot;
-    unsigned short* g_h = (L.h2lds && L.h_aliased()) ? reinterpret_cast<unsigned short*>(lds + L.off_h2())
-                                                     : (g_slot ? g_slot + L.hbytes() / 2 : nullptr);
+    unsigned short* s_h = reinterpret_cast<unsigned short*>(lds + L.off_facc());
+    unsigned short* g_h = L.h2lds ? reinterpret_cast<unsigned short*>(lds + L.off_h2())
+                                  : (Q.hslot ? Q.hslot + (size_t)blockIdx.x * (L.hbytes() / 2) : nullptr);
     const bool learned = PC >= 2 && L.learned && P.pc_mode == 2;
     const bool inkernel = learned && P.chunk == 0;  // network evaluated inside this kernel
     constexpr int NW = BLOCK / 64;
@@ -537,9 +535,7 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
             propose_gains<BLOCK>(P, B.neighs + I.a0, B.REQs + I.a0, B.elem + I.a0, s_pos, s_fnb, s_req, s_h, g_h,
                                  s_gain, &s_cred[0][0][0], N, M, nc, hasAtom ? B.comp[I.a0 + tid] : 0,
                                  hasAtom ? B.PLQs[I.a0 + tid] : make_float4(0, 0, 0, 0));
-            if (L.h_aliased()) {  // the exclusion rows were overwritten by the activations
-                stage_mask();
-            }
+            stage_mask();  // the exclusion rows were overwritten by the activations
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const int v = tid + r * BLOCK;
@@ -635,10 +631,10 @@ extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block, int learned
     return L.bytes();
 }
 
-// bf16 elements of one workgroup's global activation slot (two buffers)
+// bf16 elements of one workgroup's global activation slot (the second buffer)
 extern "C" int fc_hslot_elems(int nmax) {
     fcd::LdsLayout L{nmax, 1, 1, 1, 1, 0};
-    return L.hbytes();
+    return L.hbytes() / 2;
 }
 
 extern "C" hipError_t fc_persistent_occupancy(int block, int hb, int pc, size_t lds, int* nblocks) {
