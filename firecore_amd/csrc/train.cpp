@@ -202,9 +202,8 @@ int World::relax_learned(int nstep_max, double Fconv, bool restore_start) {
     if (restore_start)
         hip_ok(hipMemcpyAsync(d_atoms.p, d_atoms0.p, sizeof(float4) * ntotV, hipMemcpyDeviceToDevice, stream),
                "restore");
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    EventSet events;
+    hipEvent_t e0 = events.make(), e1 = events.make();
     hipEventRecord(e0, stream);
     // exact forces at the start (first network evaluation); resets status, steps and FIRE state
     if (!eval_forces()) return -1;
@@ -214,7 +213,7 @@ int World::relax_learned(int nstep_max, double Fconv, bool restore_start) {
     P.chunk = std::max(pc_every, 1);
     // events for the fork/join of the class launches
     std::vector<hipEvent_t> evs(classes.size() + 1);
-    for (auto& e : evs) hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    for (auto& e : evs) e = events.make(hipEventDisableTiming);
     auto iteration = [&](hipStream_t st) -> bool {
         if (!hip_ok(fc_launch_mlp_gains(st, &B, &P, d_gnode.p, d_gcomp.p), "gains")) return false;
         if (!hip_ok(fc_launch_zero_int(st, d_queue.p, 8), "queue reset")) return false;
@@ -234,15 +233,21 @@ int World::relax_learned(int nstep_max, double Fconv, bool restore_start) {
         return true;
     };
     const int check = 16;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    struct GraphGuard {  // the captured chunk, released on every exit path
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        ~GraphGuard() {
+            if (exec) hipGraphExecDestroy(exec);
+            if (graph) hipGraphDestroy(graph);
+        }
+    } cg;
     bool use_graph = use_graphs;
     if (use_graph) {
         use_graph = hipStreamBeginCapture(stream, hipStreamCaptureModeGlobal) == hipSuccess;
         if (use_graph) {
             const bool ok = iteration(stream);
-            use_graph = (hipStreamEndCapture(stream, &graph) == hipSuccess) && ok && graph;
-            if (use_graph) use_graph = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+            use_graph = (hipStreamEndCapture(stream, &cg.graph) == hipSuccess) && ok && cg.graph;
+            if (use_graph) use_graph = hipGraphInstantiate(&cg.exec, cg.graph, nullptr, nullptr, 0) == hipSuccess;
         }
         if (!use_graph) {
             (void)hipGetLastError();
@@ -255,7 +260,7 @@ int World::relax_learned(int nstep_max, double Fconv, bool restore_start) {
     bool ok = true;
     for (; it < max_iter && ok;) {
         for (int k = 0; k < check && it < max_iter; k++, it++) {
-            if (use_graph) ok = hip_ok(hipGraphLaunch(exec, stream), "graph launch");
+            if (use_graph) ok = hip_ok(hipGraphLaunch(cg.exec, stream), "graph launch");
             else ok = iteration(stream);
             if (!ok) break;
         }
@@ -270,11 +275,6 @@ int World::relax_learned(int nstep_max, double Fconv, bool restore_start) {
     hipEventElapsedTime(&ms, e0, e1);
     last_ms = ms;
     last_chunks = it;
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
-    for (auto& e : evs) hipEventDestroy(e);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     if (!ok) return -1;
     d_nsteps.download(nsteps, stream);
     d_status.download(status, stream);

@@ -584,18 +584,15 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
         }
         d_hslot.alloc(std::max<size_t>(tot, 1));
     }
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    EventSet events;
+    hipEvent_t e0 = events.make(), e1 = events.make();
     hipEventRecord(e0, stream);
     std::vector<hipEvent_t> starts, done;
     for (size_t k = 0; k < classes.size(); k++) {
         SizeClass& C = classes[k];
         hipStream_t st = class_streams[k % class_streams.size()];
         hipStreamWaitEvent(st, e0, 0);
-        hipEvent_t es, ev;
-        hipEventCreate(&es);
-        hipEventCreate(&ev);
+        hipEvent_t es = events.make(), ev = events.make();
         hipEventRecord(es, st);
         hipError_t err = fc_launch_relax_persistent(C.block, hb_batch ? 1 : 0, C.ngroups, C.lds, st, &B, &P, d_order.p,
                                                     qinfo.data(), d_queue.p, C.cls, C.nmax, C.mmax, C.wmax,
@@ -616,11 +613,7 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
         float cm = 0;
         hipEventElapsedTime(&cm, starts[c], done[c]);
         classes[c].last_ms = cm;
-        hipEventDestroy(starts[c]);
     }
-    for (auto ev : done) hipEventDestroy(ev);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     d_nsteps.download(nsteps, stream);
     d_status.download(status, stream);
     hip_ok(hipStreamSynchronize(stream), "relax results");
@@ -638,9 +631,8 @@ int World::relax_split(int nstep_max, double Fconv, bool restore_start) {
     fcd::RunParams P = run_params(Fconv, nstep_max);
     hipMemsetAsync(d_avel.p, 0, sizeof(float4) * ntotV, stream);
     fc_launch_reset(stream, &B, &P);
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    EventSet events;
+    hipEvent_t e0 = events.make(), e1 = events.make();
     hipEventRecord(e0, stream);
     const int chunk = 50;
     int step = 0;
@@ -660,8 +652,6 @@ int World::relax_split(int nstep_max, double Fconv, bool restore_start) {
     float ms = 0;
     hipEventElapsedTime(&ms, e0, e1);
     last_ms = ms;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     d_nsteps.download(nsteps, stream);
     d_status.download(status, stream);
     hip_ok(hipStreamSynchronize(stream), "split results");

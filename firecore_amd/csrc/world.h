@@ -22,6 +22,22 @@ struct DevArr {
     void download(std::vector<T>& h, hipStream_t s) const;
 };
 
+// HIP events of one call, destroyed on every exit path of that call.
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    EventSet() = default;
+    EventSet(const EventSet&) = delete;
+    EventSet& operator=(const EventSet&) = delete;
+    hipEvent_t make(unsigned flags = hipEventDefault) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, flags) == hipSuccess) ev.push_back(e);
+        return e;
+    }
+    ~EventSet() {
+        for (hipEvent_t e : ev) hipEventDestroy(e);
+    }
+};
+
 // Size class of the persistent kernel: all systems whose (N, M) fit one
 // workgroup shape share one launch.
 struct SizeClass {
