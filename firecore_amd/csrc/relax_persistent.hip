@@ -276,6 +276,7 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                                     j = (j >= N) ? j - N : j;
                                     const float4 pj = s_pos[j];
                                     const float4 rj = s_req[j];
+                                    float4 a = facc[lane + k - 1];  // issued with the partner rows
                                     const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
                                     float fr, ep;
                                     lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
@@ -283,7 +284,6 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                                     f.x -= d.x * fr;
                                     f.y -= d.y * fr;
                                     f.z -= d.z * fr;
-                                    float4 a = facc[lane + k - 1];
                                     a.x += d.x * fr;
                                     a.y += d.y * fr;
                                     a.z += d.z * fr;
@@ -298,12 +298,12 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                                     const unsigned m = s_mask[(k >> 5) * N + i];
                                     const float4 pj = s_pos[j];
                                     const float4 rj = s_req[j];
+                                    float4 a = facc[lane + k - 1];
                                     const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
                                     float fr, ep;
                                     lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
                                     fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
                                     f -= d * fr;
-                                    float4 a = facc[lane + k - 1];
                                     a.x += d.x * fr;
                                     a.y += d.y * fr;
                                     a.z += d.z * fr;
