@@ -138,11 +138,11 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 // s_fself / s_fng (whose recoils have been consumed) hold the star terms.
 // Star gain g_s and molecule gain g_c are P.pc_gstar / P.pc_grigid, or the
 // learned gains in s_gain (per node) and s_gain[M + c] (per component).
-template <int BLOCK, class Propose>
+template <int BLOCK>
 __device__ __forceinline__ void precondition(const RunParams& P, const float4* s_pos, float4* s_fnb, float4* s_fself,
                                              float4* s_fng, float* s_cred, float* s_csum, const float* s_gain,
                                              bool learned, int N, int M, int nc, int myc, bool hasAtom,
-                                             bool hasNode, int4 ng, int4 bk0, V3& fv0, Propose propose) {
+                                             bool hasNode, int4 ng, int4 bk0, V3& fv0) {
     constexpr int NW = BLOCK / 64;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -196,9 +196,6 @@ __device__ __forceinline__ void precondition(const RunParams& P, const float4* s
         o[8] = w.y;
         o[9] = w.z;
     }
-    // learned gains are refreshed here (exact atom forces are in s_fnb; the
-    // recoil areas s_fself/s_fng are free until the star step below)
-    propose();
     if (hasNode) {
         // star of node tid: the node (slot 0) and its bonded neighbours (slots 1..4, masked when -1)
         V3 rs[5], fs[5];

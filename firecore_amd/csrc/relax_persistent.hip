@@ -402,7 +402,7 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                 }
                 if (PC)
                     precondition<BLOCK>(P, s_pos, s_fnb, s_fself, s_fng, &s_cred[0][0][0], &s_csum[0][0], s_gain,
-                                        learned, N, M, nc, myc, hasAtom, hasNode, ng, bk[0], fv[0], [] {});
+                                        learned, N, M, nc, myc, hasAtom, hasNode, ng, bk[0], fv[0]);
                 // FIRE sums: totals per chunk of 64 consecutive vecs (vec v = tid + r*BLOCK
                 // lies in chunk wid + r*NW), added in chunk order, so that the result does
                 // not depend on the workgroup width a system is relaxed with

@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(256) k_precond(BatchDev B, RunParams P, float4
     const int myc = hasAtom ? B.comp[I.a0 + tid] : 0;
     __syncthreads();
     precondition<BLOCK>(P, s_pos, s_fnb, s_fself, s_fng, &s_cred[0][0][0], &s_csum[0][0], s_gain, learned, N, M,
-                        B.ncomp[s], myc, hasAtom, hasNode, ng, bk0, f, [] {});
+                        B.ncomp[s], myc, hasAtom, hasNode, ng, bk0, f);
     if (hasAtom) out[I.v0 + tid] = make_float4(f.x, f.y, f.z, 0.f);
     for (int v = N + tid; v < N + M; v += BLOCK) out[I.v0 + v] = B.aforce[I.v0 + v];
 }
