@@ -312,7 +312,8 @@ __device__ __forceinline__ void bs_basis_a(float u, int a, float& b, float& d) {
     }
 }
 
-__device__ __forceinline__ float grid_sample_cached(const GridDev& g, V3 p, V3 plq, float4* cache, int& key, V3& f) {
+__device__ __forceinline__ float grid_sample_cached(const GridDev& g, V3 p, V3 plq, float4* cache, int stride, int& key,
+                                                    V3& f) {
     const float ux = (p.x - g.p0x) * g.isx;
     const float uy = (p.y - g.p0y) * g.isy;
     const float uz = (p.z - g.p0z) * g.isz;
@@ -338,7 +339,7 @@ __device__ __forceinline__ float grid_sample_cached(const GridDev& g, V3 p, V3 p
         if (hit) {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const float4 v = cache[a * 4 + q];
+                const float4 v = cache[(a * 4 + q) * stride];
                 cs[q * 4 + 0] = v.x;
                 cs[q * 4 + 1] = v.y;
                 cs[q * 4 + 2] = v.z;
@@ -360,7 +361,7 @@ __device__ __forceinline__ float grid_sample_cached(const GridDev& g, V3 p, V3 p
             }
 #pragma unroll
             for (int q = 0; q < 4; q++)
-                cache[a * 4 + q] = make_float4(cs[q * 4], cs[q * 4 + 1], cs[q * 4 + 2], cs[q * 4 + 3]);
+                cache[(a * 4 + q) * stride] = make_float4(cs[q * 4], cs[q * 4 + 1], cs[q * 4 + 2], cs[q * 4 + 3]);
         }
         float ey = 0.f, fy = 0.f, fz = 0.f;
 #pragma unroll

@@ -62,8 +62,10 @@ struct BatchDev {
     // 0, snap_every, 2*snap_every, ... into snap[k * ntotV + v0 + v], k < nsnap
     float4* snap;
     int snap_every, nsnap, ntotV;
-    // per-atom GridFF window cache of the persistent kernel: 16 float4 per atom
+    // per-atom GridFF window cache of the persistent kernel: 16 float4 per atom,
+    // row r of atom a at gridcache[r * gridcache_stride + a] (coalesced over atoms)
     float4* gridcache;
+    int gridcache_stride;
     // learned preconditioner gains (written by the proposal network)
     const float* gnode;    // [sum M] star gains
     const float* gcomp;    // [nsys * PC_MAXC] molecule gains

@@ -324,8 +324,8 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                             }
                         }
                         if (P.bGridFF) {
-                            float4* gc = B.gridcache + (size_t)(I.a0 + i) * 16;
-                            grid_sample_cached(P.grid, pi, mk(plq.x, plq.y, plq.z), gc, gkey, f);
+                            grid_sample_cached(P.grid, pi, mk(plq.x, plq.y, plq.z), B.gridcache + I.a0 + i,
+                                               B.gridcache_stride, gkey, f);
                         }
                         s_fnb[i] = make_float4(f.x, f.y, f.z, 0.f);
                     }

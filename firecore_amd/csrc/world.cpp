@@ -397,6 +397,7 @@ fcd::BatchDev World::batch_dev() {
     B.gnode = d_gnode.p;
     B.gcomp = d_gcomp.p;
     B.gridcache = d_gridcache.p;
+    B.gridcache_stride = ntotA;
     B.snap = (snap_on && d_snap.p) ? d_snap.p : nullptr;
     B.snap_every = std::max(snap_every, 1);
     B.nsnap = nsnap;
