@@ -58,3 +58,25 @@ def test_host_setters_validate_their_arguments():
                 raise AssertionError("set_sched accepted %d" % bad)
     finally:
         native.set_sched(old)
+
+
+def test_persistent_lds_budget():
+    """LDS per workgroup of the persistent kernel (host-side layout function):
+    the windowed reaction rows and half exclusion rows keep three 256-atom
+    workgroups and four 192-atom workgroups within a CU's 160 KB, and the
+    learned form's padded layout always holds the first activation buffer."""
+    L = native.lib()
+    L.fc_lds_bytes.restype = ctypes.c_int
+    L.fc_hslot_elems.restype = ctypes.c_int
+    cu = 160 * 1024
+    for nmax, mmax, block, per_cu in [(256, 128, 256, 3), (192, 100, 192, 4), (128, 66, 128, 6)]:
+        wmax = ((nmax >> 1) >> 5) + 1
+        base = L.fc_lds_bytes(nmax, mmax, wmax, block, 0, 0)
+        learned = L.fc_lds_bytes(nmax, mmax, wmax, block, 1, 0)
+        assert learned >= base
+        assert per_cu * learned <= cu, (nmax, learned)
+        # first activation buffer (nmax rows of 68 bf16) fits between facc and the gains
+        facc = 16 * (nmax + mmax) + 16 * nmax * 2
+        assert learned - facc >= nmax * 68 * 2
+    for nmax in (16, 64, 100, 256):
+        assert L.fc_hslot_elems(nmax) == ((nmax * 68 * 2 + 15) & ~15) // 2
