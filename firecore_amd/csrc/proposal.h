@@ -226,7 +226,7 @@ __device__ __forceinline__ void mlp_forward(const MlpDev& M, unsigned short* h, 
 // allows, else a per-workgroup slot in global memory that stays in L2):
 // layer 1 hA -> hG, layer 2 hG -> hA, layer 3 hA -> hG, head from hG.
 template <int BLOCK, int KIN>
-__device__ __forceinline__ void mlp_layer_lean(const unsigned short* hin, const int4* nb, const unsigned short* W,
+__device__ __forceinline__ void mlp_layer_rows(const unsigned short* hin, const int4* nb, const unsigned short* W,
                                                const float* bias, int n, unsigned short* hout) {
     constexpr int NW = BLOCK / 64;
     constexpr int NK = KIN / 16;
@@ -275,14 +275,14 @@ __device__ __forceinline__ void mlp_layer_lean(const unsigned short* hin, const 
 // The global writes of a layer are made visible to the whole workgroup by
 // the fence + barrier that follow them.
 template <int BLOCK>
-__device__ __forceinline__ float2 mlp_forward_lean(const MlpDev& M, unsigned short* hA, unsigned short* hG,
+__device__ __forceinline__ float2 mlp_forward_rows(const MlpDev& M, unsigned short* hA, unsigned short* hG,
                                                    const int4* nb, int n) {
-    mlp_layer_lean<BLOCK, 16>(hA, nb, M.W1, M.b1, n, hG);
+    mlp_layer_rows<BLOCK, 16>(hA, nb, M.W1, M.b1, n, hG);
     __threadfence_block();
     __syncthreads();
-    mlp_layer_lean<BLOCK, 128>(hG, nb, M.W2, M.b2, n, hA);
+    mlp_layer_rows<BLOCK, 128>(hG, nb, M.W2, M.b2, n, hA);
     __syncthreads();
-    mlp_layer_lean<BLOCK, 128>(hA, nb, M.W3, M.b3, n, hG);
+    mlp_layer_rows<BLOCK, 128>(hA, nb, M.W3, M.b3, n, hG);
     __threadfence_block();
     __syncthreads();
     float2 o = make_float2(0.f, 0.f);
@@ -352,7 +352,7 @@ __device__ __forceinline__ void propose_gains(const RunParams& P, const int4* nb
         s_nb[tid] = nb[tid];
     }
     __syncthreads();
-    const float2 o = mlp_forward_lean<BLOCK>(P.mlp, h, hG, s_nb, N);
+    const float2 o = mlp_forward_rows<BLOCK>(P.mlp, h, hG, s_nb, N);
     if (tid < N) s_req[tid] = req[tid];
     if (tid < M) s_gain[tid] = P.pc_gstar * sigmoidf_(o.x);
     for (int c = 0; c < nc; c++) {
