@@ -270,10 +270,10 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                                 const unsigned m = s_mask[w * N + i];
                                 const int k0 = max(1, w << 5);
                                 const int k1 = min(K, (w << 5) + 31);
+                                int j = i + k0;
+                                j = (j >= N) ? j - N : j;
 #pragma unroll 4
-                                for (int k = k0; k <= k1; k++) {
-                                    int j = i + k;
-                                    j = (j >= N) ? j - N : j;
+                                for (int k = k0; k <= k1; k++, j = (j + 1 == N) ? 0 : j + 1) {
                                     const float4 pj = s_pos[j];
                                     const float4 rj = s_req[j];
                                     float4 a = facc[lane + k - 1];  // issued with the partner rows
