@@ -127,6 +127,34 @@ def test_trajectory_parity_first_steps(nat, orc, path):
         assert d < 2e-4, (s, d)
 
 
+@pytest.mark.parametrize("path", [0, 1])
+def test_hbond_trajectory_parity(nat, orc, path):
+    """H-bond charges (REQ.w of opposite signs on donors and acceptors) select
+    the kernels' H-bond variant of the pair term (getLJQH, Forces.h:341-360);
+    its FIRE trajectory follows the FP64 oracle over the first 40 steps."""
+    P("case: H-bond trajectory path", path)
+    rng = np.random.default_rng(7)
+    mols = [nat.gen_molecule(n, s, 0.1, s + 1) for n, s in [(24, 3), (70, 4), (150, 5)]]
+    for m in mols:
+        R = np.array(m["REQs"], dtype=np.float64).reshape(-1, 4)
+        R[:, 3] = rng.choice([0.0, 0.5, -0.5], size=len(R))
+        m["REQs"] = R
+    nat.set_force_limit(0.0)
+    nat.set_opt(*FIRE_REF)
+    nat.init_systems(mols)
+    nat.relax(40, 1e-12, path, True, True)
+    B = nat.Batch()
+    assert (B.status == nat.ST_MAXSTEP).all() and (B.nsteps == 40).all()
+    for s, m in enumerate(mols):
+        n, apos, E, traj = orc.relax(m, FIRE_REF, 1e-12, 40, 0, ntraj=40)
+        d = np.abs(B.positions(s) - traj[-1]).max()
+        assert d < 2e-4, (s, d)
+        # the H-bond term moves the trajectory by far more than the tolerance
+        m0 = dict(m, REQs=np.array(m["REQs"]) * np.array([1.0, 1.0, 1.0, 0.0]))
+        _, _, _, traj0 = orc.relax(m0, FIRE_REF, 1e-12, 40, 0, ntraj=40)
+        assert np.abs(traj0[-1] - traj[-1]).max() > 20 * 2e-4, s
+
+
 def test_relax_config1_replicas(nat, orc):
     """1024 replicas of one 24-atom molecule, vacuum, FIRE to max|f| < 1e-3."""
     P("case: config-1 relax")
