@@ -110,6 +110,19 @@ def test_forces_ring_corrections(nat, orc):
     check_forces(nat, orc, nat.Batch())
 
 
+def test_forces_ring_corrections_persistent(nat, orc):
+    """The persistent kernel's forces and energy at the start geometry (a
+    zero-step relaxation) on the 4-ring, whose 1-3 pairs reached through two
+    nodes go through the weighted pair corrections."""
+    P("case: ring topology, persistent kernel")
+    nat.set_force_limit(0.0)
+    nat.init_systems([ring_molecule(), nat.gen_molecule(20, 3, 0.1, 4)])
+    nat.relax(0, 1e-12, 0, True, True)
+    B = nat.Batch()
+    assert (B.status == nat.ST_MAXSTEP).all() and (B.nsteps == 0).all()
+    check_forces(nat, orc, B)
+
+
 @pytest.mark.parametrize("path", [0, 1])
 def test_trajectory_parity_first_steps(nat, orc, path):
     """FP32 device FIRE follows the FP64 trajectory over the first 40 steps."""
