@@ -110,6 +110,30 @@ def test_forces_ring_corrections(nat, orc):
     check_forces(nat, orc, nat.Batch())
 
 
+@pytest.mark.parametrize("path", [0, 1])
+def test_gridff_trajectory_parity(nat, orc, path):
+    """Six systems of the headline workload over the NaCl GridFF follow the
+    FP64 trajectory for 40 FIRE steps (per-vec clamp 10 eV/A on both sides);
+    on the persistent path the later steps read the per-atom window cache."""
+    P("case: GridFF trajectory path", path)
+    fire = tuple(FIRE_REF) + (10.0,)
+    nat.set_opt(*FIRE_REF)
+    nat.set_force_limit(10.0)
+    try:
+        nat.init_workload(1, 6, 10001, 16, 120, 0.1, "synth:NaCl", 0.1)
+        nat.relax(40, 1e-12, path, True, True)
+        B = nat.Batch()
+        g = nat.grid_info()
+        assert (B.nsteps == 40).all()
+        for s in range(6):
+            m = nat.gen_workload_system(1, s, 10001, 16, 120, 0.1, g)
+            n, apos, E, traj = orc.relax(m, fire, 1e-12, 40, 0, grid=g, ntraj=40)
+            d = np.abs(B.positions(s) - traj[-1]).max()
+            assert d < 2e-4, (s, d)
+    finally:
+        nat.set_force_limit(0.0)
+
+
 def test_forces_ring_corrections_persistent(nat, orc):
     """The persistent kernel's forces and energy at the start geometry (a
     zero-step relaxation) on the 4-ring, whose 1-3 pairs reached through two
