@@ -170,7 +170,7 @@ def main():
                     help="copy a 256 MiB buffer once first (the known-size kernel that scales memory counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs (--gpus %d) are not part of it" % args.gpus,
+        print("bench.py: the benchmark runs on one GPU; multi-GPU runs (--gpus %d) are not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 

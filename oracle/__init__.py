@@ -1,7 +1,7 @@
 """FP64 CPU oracle of the multi-system MMFF relaxation path (test infrastructure).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
-package, and only as the checker or the timed CPU baseline.  The product
+package, and only to check results or as the timed CPU baseline.  The product
 library (firecore_amd/libMMFFmulti_lib.so) never loads it.
 
 Two back-ends with one call signature:

@@ -3,7 +3,7 @@
 // TEST INFRASTRUCTURE ONLY.  This is a plain restatement, in double
 // precision, of the reference CPU path that the GPU library must reproduce.
 // Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
-// may load it, and only as the checker / the timed CPU baseline; the product
+// may load it, and only to check results or as the timed CPU baseline; the
 // library never calls it.
 //
 // What each function restates (reference = ProkopHapala/FireCore):
