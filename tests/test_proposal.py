@@ -48,6 +48,21 @@ def test_oracle_backward_matches_finite_differences():
         assert abs(fd - g[p]) <= 1e-6 * (1 + abs(fd)), (p, fd, g[p])
 
 
+def test_training_start_is_the_constant_gain_setting():
+    """train_proposal's initial parameters start every gain at the constant
+    setting (g0 = 2 per star, 63 per molecule, within 10 %) under the ceilings: the head's
+    biases are the logits of g0/gmax, its weights are small."""
+    from firecore_amd import train_proposal as tp
+    th = tp.initial_params(3, tp.GSTAR, tp.GRIGID).astype(np.float64)
+    rng = np.random.default_rng(0)
+    N = 30
+    x = rng.normal(size=(N, 16))
+    nb = [[j for j in (i - 1, i + 1) if 0 <= j < N] for i in range(N)]
+    o, _ = mlp.forward(th, x, nb)
+    g_node, g_comp = mlp.gains(o, 10, np.zeros(N, np.int64), tp.GSTAR, tp.GRIGID)
+    assert np.allclose(g_node, tp.G0_STAR, rtol=0.1) and np.allclose(g_comp[:1], tp.G0_RIGID, rtol=0.1)
+
+
 def test_bf16_rounding_matches_torch():
     torch = pytest.importorskip("torch")
     rng = np.random.default_rng(2)
