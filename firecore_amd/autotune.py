@@ -1,0 +1,90 @@
+"""Autotuner of the persistent relaxation kernel's launch plan.
+
+The plan has three free parts, each picked on a calibration batch whose
+systems come from seeds disjoint from the batch that is then relaxed (so no
+choice is made with knowledge of the relaxed systems themselves):
+
+* the cost model of the queue order: a system's cost is steps(N) * N^2 with
+  steps(N) = c0 + c1 * N, fitted by least squares to the step counts of the
+  calibration relaxation (per form: plain, constant gains and learned gains
+  converge in very different numbers of steps);
+* the scheduling mode of the four size-class launches (set_sched: 1 = every
+  class at full device width, 2 = twice the resident width, 0 = slots split
+  by estimated time with stealing), timed on the calibration batch;
+* the set of workgroup widths (64, 128, 192, 256 threads; a system goes to
+  the narrowest enabled width that holds it), timed the same way.
+
+The result is a plain dict (stored by bench.py with its output); apply() sets
+it on the library.  Calibration relaxes the batch a few times, about a second
+of device time for the default 512 systems.
+"""
+import time
+
+import numpy as np
+
+from . import native
+
+
+def fit_steps(N, steps):
+    """Least-squares (c0, c1) of steps ~ c0 + c1*N, clipped to positive costs."""
+    A = np.stack([np.ones(len(N)), np.asarray(N, np.float64)], 1)
+    c, *_ = np.linalg.lstsq(A, np.asarray(steps, np.float64), rcond=None)
+    c0, c1 = float(c[0]), float(c[1])
+    if c0 + c1 * float(np.min(N)) < 1.0:  # keep every estimate positive
+        c0 = 1.0 - c1 * float(np.min(N))
+    return c0, c1
+
+
+def _time_pass(nstep_max, fconv, reps=2):
+    ms = []
+    for _ in range(reps):
+        native.relax(nstep_max, fconv, 0, True, False)
+        ms.append(native.last_ms())
+    return float(np.median(ms))
+
+
+def calibrate(setup_form, kind=1, nsys=512, seed0=90001, nmin=16, nmax=256, surf="synth:NaCl", nstep_max=10000,
+              fconv=1e-3, log=None):
+    """Tune the plan for one form.  setup_form() installs the form (precond
+    mode, weights) after the calibration batch is built.  Returns the plan."""
+    t0 = time.perf_counter()
+    native.init_workload(kind, nsys, seed0, nmin, nmax, 0.1, surf, 0.1)
+    setup_form()
+    native.set_cost_model(0, 1000.0, 20.0)
+    native.set_sched(1)
+    native.set_class_mask(0xF)
+    native.relax(nstep_max, fconv, 0, True, True)
+    B = native.Batch()
+    N = np.array(B.info[:, 0])
+    st = np.array(B.nsteps)
+    ok = np.array(B.status) == native.ST_CONVERGED
+    c0, c1 = fit_steps(N[ok], st[ok]) if ok.sum() >= 8 else (1000.0, 20.0)
+    native.set_cost_model(0, c0, c1)
+    trials = {}
+    for sched in (1, 2, 0):
+        native.set_sched(sched)
+        trials["sched%d" % sched] = _time_pass(nstep_max, fconv)
+    best_sched = int(min(trials, key=trials.get)[5:])
+    native.set_sched(best_sched)
+    widths = {}
+    for mask in (0xF, 0xB, 0xD, 0x9):  # bit c enables width 64*(c+1); 256 always on
+        native.set_class_mask(mask)
+        widths[mask] = _time_pass(nstep_max, fconv)
+    best_mask = int(min(widths, key=widths.get))
+    native.set_class_mask(best_mask)
+    plan = dict(cost_c0=round(c0, 3), cost_c1=round(c1, 4), sched=best_sched, class_mask=best_mask,
+                widths=[64 * (c + 1) for c in range(4) if (best_mask >> c) & 1],
+                calib_systems=nsys, calib_seeds="%d..%d" % (seed0, seed0 + nsys - 1),
+                calib_ms={k: round(v, 2) for k, v in trials.items()},
+                calib_mask_ms={hex(k): round(v, 2) for k, v in widths.items()},
+                calib_s=round(time.perf_counter() - t0, 2))
+    if log:
+        log("autotune: %s" % plan)
+    return plan
+
+
+def apply(plan):
+    """Install a plan (no feedback from earlier relaxations of the batch)."""
+    native.set_cost_model(0, plan["cost_c0"], plan["cost_c1"])
+    native.set_sched(int(plan["sched"]))
+    native.set_class_mask(int(plan.get("class_mask", 0xF)))

@@ -736,6 +736,14 @@ int init_workload(int kind, int nSys, long long seed0, int nmin, int nmax, doubl
     return W.ntotA;
 }
 
+int attach_surface(const char* surf_name, double gridStep) {
+    World& W = world();
+    if (!surf_name || !surf_name[0] || W.nSys == 0) return -1;
+    fc::GridFFHost g;
+    if (!setup_surface(surf_name, gridStep > 0 ? gridStep : 0.1, g)) return -1;
+    return W.set_grid(g) ? 0 : -1;
+}
+
 // Relax the whole batch on the device.
 //   path 0: persistent kernel; 1: split-phase kernels
 //   restore_start: restart from the positions of the last init/upload
@@ -750,6 +758,47 @@ int relax(int nstepMax, double Fconv, int path, bool restore_start, bool bDownlo
 }
 
 double get_last_ms() { return world().last_ms; }
+
+int set_cost_model(int feedback, double c0, double c1) {
+    World& W = world();
+    if (!std::isfinite(c0) || !std::isfinite(c1)) return -1;
+    W.cost_feedback = feedback != 0;
+    W.cost_c0 = c0;
+    W.cost_c1 = c1;
+    return 0;
+}
+
+int set_class_mask(int mask) {
+    World& W = world();
+    if (mask < 0 || mask > 0xF) return -1;
+    const int old = W.class_mask;
+    W.class_mask = mask | 0x8;
+    if (W.dev_ready && W.nSys > 0) W.plan_classes();
+    return old;
+}
+
+int set_record_times(int on) {
+    World& W = world();
+    const int old = W.record_times ? 1 : 0;
+    W.record_times = on != 0;
+    return old;
+}
+
+// Per-system start and end of the last persistent relaxation, in microseconds
+// after the earliest start (device wall clock), into out[2*s], out[2*s+1].
+int get_system_times(double* out) {
+    World& W = world();
+    if (!W.record_times || !W.d_systime.p || W.nSys == 0) return -1;
+    std::vector<long long> t((size_t)2 * W.nSys);
+    W.d_systime.download(t, W.stream);
+    if (!fcw::hip_ok(hipStreamSynchronize(W.stream), "system times")) return -1;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0) != hipSuccess || khz <= 0) return -1;
+    long long t0 = t[0];
+    for (int s = 0; s < W.nSys; s++) t0 = std::min(t0, t[2 * s]);
+    for (size_t i = 0; i < t.size(); i++) out[i] = (double)(t[i] - t0) * 1000.0 / khz;
+    return 0;
+}
 
 // Size classes of the persistent kernel (one launch each): per class
 // {block, nsys, nmax, mmax, ngroups, lds_bytes} into info6[6*k] and the device

@@ -27,6 +27,13 @@ __device__ __forceinline__ V3 cross(V3 a, V3 b) {
     return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
+// Wave shift by one lane (DPP wave_shl:1 with bound_ctrl): lane l receives
+// lane l+1's value, lane 63 receives 0.  Needs all 64 lanes active.
+__device__ __forceinline__ float dpp_shl1_zero(float src) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, src), 0x130, 0xF, 0xF,
+                                                                 true));
+}
+
 // Switches of the bonded terms (setSwitches_multi, MMFFmulti_lib.cpp:316-329).
 struct FFSwitch {
     int doBonds, doPiPiI, doPiSigma, doAngles;

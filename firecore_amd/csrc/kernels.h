@@ -66,6 +66,9 @@ struct BatchDev {
     // row r of atom a at gridcache[r * gridcache_stride + a] (coalesced over atoms)
     float4* gridcache;
     int gridcache_stride;
+    // optional per-system wall-clock record (null = off): start and end of the
+    // workgroup's relaxation of system s at systime[2s], systime[2s+1]
+    long long* systime;
     // learned preconditioner gains (written by the proposal network)
     const float* gnode;    // [sum M] star gains
     const float* gcomp;    // [nsys * PC_MAXC] molecule gains

@@ -521,13 +521,15 @@ __global__ void k_zero_int(int* a, int n) {
     if (i < n) a[i] = 0;
 }
 
-// Number of systems still running, into *out.
-__global__ void k_count_running(const int* status, int nsys, int* out) {
+// Number of systems still running, into *out (systems with skip[s] != 0 not
+// counted; skip may be null).
+__global__ void k_count_running(const int* status, int nsys, int* out, const int* skip) {
     __shared__ int cnt;
     if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
     int c = 0;
-    for (int s = threadIdx.x; s < nsys; s += blockDim.x) c += (status[s] == ST_RUNNING) ? 1 : 0;
+    for (int s = threadIdx.x; s < nsys; s += blockDim.x)
+        c += (status[s] == ST_RUNNING && !(skip && skip[s])) ? 1 : 0;
     atomicAdd(&cnt, c);
     __syncthreads();
     if (threadIdx.x == 0) out[0] = cnt;
@@ -556,7 +558,7 @@ extern "C" hipError_t fc_launch_zero_int(hipStream_t st, int* a, int n) {
     hipLaunchKernelGGL(fcd::k_zero_int, dim3((n + 255) / 256), dim3(256), 0, st, a, n);
     return hipGetLastError();
 }
-extern "C" hipError_t fc_launch_count_running(hipStream_t st, const int* status, int nsys, int* out) {
-    hipLaunchKernelGGL(fcd::k_count_running, dim3(1), dim3(1024), 0, st, status, nsys, out);
+extern "C" hipError_t fc_launch_count_running(hipStream_t st, const int* status, int nsys, int* out, const int* skip) {
+    hipLaunchKernelGGL(fcd::k_count_running, dim3(1), dim3(1024), 0, st, status, nsys, out, skip);
     return hipGetLastError();
 }

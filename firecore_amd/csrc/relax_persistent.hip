@@ -143,6 +143,7 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
         __syncthreads();
         if (q < 0) break;
         const int s = q;
+        if (B.systime && tid == 0 && P.chunk == 0) B.systime[2 * s] = (long long)wall_clock64();
         const SysInfo I = B.info[s];
         const int N = I.natoms, M = I.nnode, V = N + M;
         const int W = (N + 31) >> 5;         // exclusion words per atom in memory
@@ -201,7 +202,10 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
         // with the forces and velocities parked in memory, and the segment
         // resumes that step at the preconditioner.  Nothing of the step loop is
         // live across the network, so neither part adds to the other's registers.
-        bool fresh = false;  // gains of the current step proposed: resume it after its force assembly
+        // gains of the current step proposed: resume it after its force assembly.
+        // A chunk of the chunked learned form resumes the step it was suspended
+        // at, whose exact forces it parked (the first chunk computes its own)
+        bool fresh = learned && !inkernel && step > 0;
         for (;;) {
             bool propose_now = false;
             // ---- per-thread state of this segment: atom tid, node tid, vecs tid and tid+BLOCK
@@ -233,10 +237,6 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                 }
             }
             for (;;) {
-                if (step >= step_end) {  // chunk done: suspend (state written back below)
-                    suspended = true;
-                    break;
-                }
                 if (B.snap && step % B.snap_every == 0 && step / B.snap_every < B.nsnap) {
                     float4* dst = B.snap + (size_t)(step / B.snap_every) * B.ntotV + I.v0;
                     for (int v = tid; v < V; v += BLOCK) dst[v] = s_pos[v];
@@ -244,73 +244,90 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                 float vf = 0.f, vv = 0.f, ff = 0.f, fm2 = 0.f;
                 if (!fresh) {
                     // ================= A: forces (no energies inside the step loop)
-                    // Half-ring pair schedule (Newton's third law): thread i takes the
-                    // partners j = (i+k) mod N, k = 1..N/2, so every unordered pair is
-                    // computed once.  The reaction -f_ij goes to the wave's private
-                    // accumulator row at entry lane+k-1 (the partners of a wave form
-                    // a window that slides with k): within one instruction the lanes
-                    // of a wave hold distinct entries, and no other wave writes that row.
+                    // Half-ring pair schedule (Newton's third law): the thread of atom i
+                    // takes the partners j = (i+k) mod N, k = 1..N/2, so every unordered
+                    // pair is computed once.  The partners of a wave at offset k are the
+                    // 64 consecutive atoms 64w+k .. 64w+63+k (mod N), a window that slides
+                    // by one atom per offset.  The reaction on each partner is accumulated
+                    // in registers and moves down one lane per offset with the window
+                    // (DPP wave shift; the atom entering at lane 63 starts from zero); the
+                    // reaction leaving at lane 0 is written to the wave's reaction row.
+                    // Entry e of wave w's row holds the reaction on atom (64w+1+e) mod N:
+                    // entries 0..N/2-2 are written as they leave, the rest by all 64 lanes
+                    // after the last offset.  Every entry is written once and read after
+                    // the block barrier: no clearing, no atomics, no lane ordering.
                     const int nwAct = (N + 63) >> 6;
-                    const int nrow = 64 + (N >> 1);  // entries a wave's row may receive
-                    float4* facc = s_facc + wid * L.frow();
-                    if (wid < nwAct)
-                        for (int j = lane; j < nrow; j += 64) facc[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    // the row is shared by the lanes of the wave: keep every LDS access to it
-                    // in program order (the compiler only sees each lane's own addresses)
-                    __builtin_amdgcn_wave_barrier();
+                    const int Kl = N >> 1;          // last offset of the half ring
+                    const int nrow = 63 + Kl;       // entries of a wave's reaction row
+                    const int wu = __builtin_amdgcn_readfirstlane(wid);
+                    V3 f = zero3();
+                    // lanes without an atom carry the window only: their own atom sits far
+                    // away with zero parameters, so each of their pair forces is exactly 0
+                    const V3 pi = hasAtom ? xyz(s_pos[tid]) : mk(1e4f, 1e4f, 1e4f);
+                    const float4 ri = hasAtom ? s_req[tid] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    const float kQi = COULOMB_K * ri.z;
+                    if (P.bNonBonded && wu < nwAct && N >= 2) {
+                        float4* facc = s_facc + wu * L.frow();
+                        const int ic = hasAtom ? tid : 0;
+                        unsigned j = (unsigned)((wu * 64 + lane + 1) % N);  // partner at offset 1
+                        V3 acc = zero3();
+                        // one offset: the pair (i, j) with partner rows (pj, rj), excluded when
+                        // bit k of the offset's mask word is set; reads the next offset's
+                        // partner rows into (pn, rn) ahead of the arithmetic
+                        auto pair_step = [&](unsigned mword, int k, const float4& pj, const float4& rj, float4& pn,
+                                             float4& rn) {
+                            j = min(j + 1, j + 1 - (unsigned)N);  // (j+1) mod N
+                            pn = s_pos[j];
+                            rn = s_req[j];
+                            const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
+                            float fr, ep;
+                            lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+                            fr = ((mword >> (k & 31)) & 1u) ? 0.f : fr;
+                            f.x = __builtin_fmaf(-d.x, fr, f.x);
+                            f.y = __builtin_fmaf(-d.y, fr, f.y);
+                            f.z = __builtin_fmaf(-d.z, fr, f.z);
+                            acc.x = __builtin_fmaf(d.x, fr, acc.x);
+                            acc.y = __builtin_fmaf(d.y, fr, acc.y);
+                            acc.z = __builtin_fmaf(d.z, fr, acc.z);
+                        };
+                        // the window moves on: the reaction of lane 0's partner leaves
+                        auto shift = [&](int k) {
+                            if (lane == 0) facc[k - 1] = make_float4(acc.x, acc.y, acc.z, 0.f);
+                            acc.x = dpp_shl1_zero(acc.x);
+                            acc.y = dpp_shl1_zero(acc.y);
+                            acc.z = dpp_shl1_zero(acc.z);
+                        };
+                        float4 pA = s_pos[j], rA = s_req[j], pB, rB;
+                        // offsets 1..Kl-1, two per iteration (partner rows alternate A/B)
+                        for (int w = 0; (w << 5) < Kl; w++) {
+                            const unsigned m = s_mask[w * N + ic];
+                            const int k0 = max(1, w << 5);
+                            const int k1 = min(Kl - 1, (w << 5) + 31);
+                            int k = k0;
+                            for (; k < k1; k += 2) {
+                                pair_step(m, k, pA, rA, pB, rB);
+                                shift(k);
+                                pair_step(m, k + 1, pB, rB, pA, rA);
+                                shift(k + 1);
+                            }
+                            if (k == k1) {
+                                pair_step(m, k, pA, rA, pB, rB);
+                                shift(k);
+                                pA = pB;
+                                rA = rB;
+                            }
+                        }
+                        // last offset Kl: for even N the pair (i, i+N/2) is taken by the lower half only
+                        {
+                            unsigned m = s_mask[(Kl >> 5) * N + ic];
+                            if (!(N & 1) && tid >= Kl) m |= 1u << (Kl & 31);
+                            pair_step(m, Kl, pA, rA, pB, rB);
+                        }
+                        facc[lane + Kl - 1] = make_float4(acc.x, acc.y, acc.z, 0.f);
+                    }
                     if (hasAtom) {
                         const int i = tid;
-                        const V3 pi = xyz(s_pos[i]);
-                        const float4 ri = s_req[i];
-                        const float kQi = COULOMB_K * ri.z;
-                        V3 f = zero3();
                         if (P.bNonBonded) {
-                            const int K = (N - 1) >> 1;
-                            for (int w = 0; (w << 5) <= K; w++) {
-                                const unsigned m = s_mask[w * N + i];
-                                const int k0 = max(1, w << 5);
-                                const int k1 = min(K, (w << 5) + 31);
-                                int j = i + k0;
-                                j = (j >= N) ? j - N : j;
-#pragma unroll 4
-                                for (int k = k0; k <= k1; k++, j = (j + 1 == N) ? 0 : j + 1) {
-                                    const float4 pj = s_pos[j];
-                                    const float4 rj = s_req[j];
-                                    float4 a = facc[lane + k - 1];  // issued with the partner rows
-                                    const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                                    float fr, ep;
-                                    lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
-                                    fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
-                                    f.x -= d.x * fr;
-                                    f.y -= d.y * fr;
-                                    f.z -= d.z * fr;
-                                    a.x += d.x * fr;
-                                    a.y += d.y * fr;
-                                    a.z += d.z * fr;
-                                    facc[lane + k - 1] = a;
-                                    __builtin_amdgcn_wave_barrier();
-                                }
-                            }
-                            if (!(N & 1)) {  // k = N/2 pairs (i, i+N/2): taken by the lower half only
-                                const int k = N >> 1;
-                                if (i < k) {
-                                    const int j = i + k;
-                                    const unsigned m = s_mask[(k >> 5) * N + i];
-                                    const float4 pj = s_pos[j];
-                                    const float4 rj = s_req[j];
-                                    float4 a = facc[lane + k - 1];
-                                    const V3 d = mk(pj.x - pi.x, pj.y - pi.y, pj.z - pi.z);
-                                    float fr, ep;
-                                    lj_pair<HB>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
-                                    fr = ((m >> (k & 31)) & 1u) ? 0.f : fr;
-                                    f -= d * fr;
-                                    a.x += d.x * fr;
-                                    a.y += d.y * fr;
-                                    a.z += d.z * fr;
-                                    facc[lane + k - 1] = a;
-                                    __builtin_amdgcn_wave_barrier();
-                                }
-                            }
                             // weighted corrections (ring topologies; empty for trees)
                             for (int c = 0; c < I.ncorr; c++) {
                                 const PairCorrDev pc = B.corr[I.corr0 + c];
@@ -353,13 +370,13 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                             const bool isPi = v >= N;
                             if (!isPi) {
                                 f = xyz(s_fnb[v]);
-                                for (int w = 0; w < nwAct; w++) {  // atom v sits at entry (v-64w-1) mod N (+N)
-                                    int e = v - 64 * w - 1;
-                                    e = (e < 0) ? e + N : e;
-                                    const float4* row = s_facc + w * L.frow();
-                                    if (e < nrow) f += xyz(row[e]);
-                                    if (e + N < nrow) f += xyz(row[e + N]);
-                                }
+                                if (P.bNonBonded && N >= 2)
+                                    for (int w = 0; w < nwAct; w++) {  // atom v: entries (v-64w-1) mod N + multiples of N
+                                        int e = (v - 64 * w - 1) % N;
+                                        e = (e < 0) ? e + N : e;
+                                        const float4* row = s_facc + w * L.frow();
+                                        for (; e < nrow; e += N) f += xyz(row[e]);
+                                    }
                                 if (v < M) f += xyz(s_fself[v]);
                             } else {
                                 f = xyz(s_fself[M + (v - N)]);
@@ -389,6 +406,13 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                             fm2 = fmaxf(fm2, dot(fv[r], fv[r]));
                         }
                     }
+                }
+                if (!fresh && step >= step_end) {
+                    // chunk done: suspend with the exact forces of the current
+                    // geometry (written back below), so that the network between
+                    // chunks sees positions and forces of the same step
+                    suspended = true;
+                    break;
                 }
                 if (PC == 2 && inkernel && !fresh && step % P.pc_every == 0) {  // uniform across the block
                     if (hasAtom) s_fnb[tid] = make_float4(fv[0].x, fv[0].y, fv[0].z, 0.f);
@@ -556,10 +580,13 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
             if (v < V) {
                 B.atoms[I.v0 + v] = s_pos[v];
                 B.avel[I.v0 + v] = make_float4(vel[r].x, vel[r].y, vel[r].z, 0.f);
-                const V3 fo = (PC && v < N) ? xyz(s_fnb[v]) : fv[r];
+                // preconditioned forms keep the exact atom forces in s_fnb; a suspended
+                // chunk holds them, assembled and not yet preconditioned, in fv
+                const V3 fo = (PC && v < N && !suspended) ? xyz(s_fnb[v]) : fv[r];
                 B.aforce[I.v0 + v] = make_float4(fo.x, fo.y, fo.z, 0.f);
             }
         }
+        if (B.systime && tid == 0 && P.chunk == 0) B.systime[2 * s + 1] = (long long)wall_clock64();
         __syncthreads();
     }
 }

@@ -245,6 +245,19 @@ __global__ void k_reset_state(BatchDev B, RunParams P) {
     B.nsteps[s] = 0;
     B.fire[s] = make_float4(P.fire.dt_max, P.fire.damp_max, 0.f, 0.f);
 }
+// The same for the systems with only[s] != 0 (velocities zeroed too); the
+// others keep their state, so the split kernels skip the finished ones.
+__global__ void __launch_bounds__(256) k_reset_subset(BatchDev B, RunParams P, const int* only) {
+    const int s = blockIdx.x;
+    if (!only[s]) return;
+    const SysInfo I = B.info[s];
+    for (int v = threadIdx.x; v < I.natoms + I.nnode; v += blockDim.x) B.avel[I.v0 + v] = make_float4(0, 0, 0, 0);
+    if (threadIdx.x == 0) {
+        B.status[s] = ST_RUNNING;
+        B.nsteps[s] = 0;
+        B.fire[s] = make_float4(P.fire.dt_max, P.fire.damp_max, 0.f, 0.f);
+    }
+}
 __global__ void k_zero_f4(float4* a, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = make_float4(0, 0, 0, 0);
@@ -308,6 +321,12 @@ extern "C" hipError_t fc_launch_split_step(hipStream_t st, const fcd::BatchDev* 
 
 extern "C" hipError_t fc_launch_reset(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P) {
     hipLaunchKernelGGL(fcd::k_reset_state, dim3((B->nsys + 255) / 256), dim3(256), 0, st, *B, *P);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t fc_launch_reset_subset(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P,
+                                             const int* only) {
+    hipLaunchKernelGGL(fcd::k_reset_subset, dim3(B->nsys), dim3(256), 0, st, *B, *P, only);
     return hipGetLastError();
 }
 

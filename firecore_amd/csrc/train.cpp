@@ -47,7 +47,8 @@ extern "C" hipError_t fc_launch_gather_snap(hipStream_t st, const fcd::BatchDev*
 extern "C" hipError_t fc_launch_mlp_gains(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P, float* gnode,
                                           float* gcomp);
 extern "C" hipError_t fc_launch_zero_int(hipStream_t st, int* a, int n);
-extern "C" hipError_t fc_launch_count_running(hipStream_t st, const int* status, int nsys, int* out);
+extern "C" hipError_t fc_launch_count_running(hipStream_t st, const int* status, int nsys, int* out,
+                                              const int* skip);
 extern "C" hipError_t fc_launch_zero_f4(hipStream_t st, float4* a, int n);
 extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups, size_t lds_bytes, hipStream_t stream,
                                                  const fcd::BatchDev* B, const fcd::RunParams* P, const int* order,
@@ -190,9 +191,8 @@ int World::relax_learned(int nstep_max, double Fconv, bool restore_start) {
         std::fprintf(stderr, "[libMMFFmulti_lib] learned relaxation: no proposal weights installed\n");
         return -1;
     }
-    if (maxN > 256 || maxM > 256 || !ring_ok) {
-        std::fprintf(stderr, "[libMMFFmulti_lib] learned relaxation needs systems of <= 256 atoms with symmetric "
-                             "exclusions\n");
+    if (!ring_ok) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] learned relaxation needs symmetric exclusions\n");
         return -1;
     }
     d_gnode.alloc(std::max(ntotM, 1));
@@ -202,6 +202,13 @@ int World::relax_learned(int nstep_max, double Fconv, bool restore_start) {
     if (restore_start)
         hip_ok(hipMemcpyAsync(d_atoms.p, d_atoms0.p, sizeof(float4) * ntotV, hipMemcpyDeviceToDevice, stream),
                "restore");
+    // systems too large for every class are relaxed afterwards (relax_unclassed)
+    // and not counted by the chunk loop
+    if (n_unclassed > 0) {
+        std::vector<int> skip(nSys);
+        for (int s = 0; s < nSys; s++) skip[s] = sys_class[s] < 0 ? 1 : 0;
+        d_only.upload(skip, stream);
+    }
     EventSet events;
     hipEvent_t e0 = events.make(), e1 = events.make();
     hipEventRecord(e0, stream);
@@ -264,7 +271,7 @@ int World::relax_learned(int nstep_max, double Fconv, bool restore_start) {
             else ok = iteration(stream);
             if (!ok) break;
         }
-        fc_launch_count_running(stream, d_status.p, nSys, d_count.p);
+        fc_launch_count_running(stream, d_status.p, nSys, d_count.p, n_unclassed > 0 ? d_only.p : nullptr);
         d_count.download(cnt, stream);
         if (!hip_ok(hipStreamSynchronize(stream), "learned relax")) ok = false;
         if (cnt[0] == 0) break;
@@ -276,6 +283,7 @@ int World::relax_learned(int nstep_max, double Fconv, bool restore_start) {
     last_ms = ms;
     last_chunks = it;
     if (!ok) return -1;
+    if (n_unclassed > 0 && !relax_unclassed(nstep_max, Fconv)) return -1;
     d_nsteps.download(nsteps, stream);
     d_status.download(status, stream);
     hip_ok(hipStreamSynchronize(stream), "learned relax results");

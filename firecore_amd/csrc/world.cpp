@@ -26,6 +26,8 @@ extern "C" hipError_t fc_launch_reset(hipStream_t st, const fcd::BatchDev* B, co
 extern "C" hipError_t fc_launch_precond(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P, float4* out,
                                         const float* gnode, const float* gcomp);
 extern "C" hipError_t fc_persistent_occupancy(int block, int hb, int pc, size_t lds, int* nblocks);
+extern "C" hipError_t fc_launch_reset_subset(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P,
+                                             const int* only);
 
 namespace fcw {
 
@@ -68,6 +70,7 @@ template struct DevArr<unsigned>;
 template struct DevArr<unsigned short>;
 template struct DevArr<float>;
 template struct DevArr<double>;
+template struct DevArr<long long>;
 template struct DevArr<fcd::SysInfo>;
 template struct DevArr<fcd::PairCorrDev>;
 
@@ -163,6 +166,15 @@ void World::build_packed() {
     Esys.assign(nSys, 0.0);
     Fmax.assign(nSys, 0.f);
     for (int s = 0; s < nSys; s++) pack_system(s, true, false, false, true);
+    // The reference's Python view maps gpu_neighs / gpu_neighCell as
+    // (nSys, nvecs, 4) (pyBall/MMFF_multi.py:233-235) although they hold
+    // natoms rows per system: for uniform batches the host mirrors are padded
+    // with -1 rows to nSys*nvecs so that such a view stays inside the buffer.
+    if (nvecs_uniform > 0) {
+        const size_t rows = std::max((size_t)ntotA, (size_t)nSys * nvecs_uniform);
+        neighs.resize(rows, make_int4(-1, -1, -1, -1));
+        neighCell.resize(rows, make_int4(-1, -1, -1, -1));
+    }
     rebuild_exclusions();
     // template views
     ndims[0] = ffl.nvecs() * 3;
@@ -396,6 +408,7 @@ fcd::BatchDev World::batch_dev() {
     B.elem = d_elem.p;
     B.gnode = d_gnode.p;
     B.gcomp = d_gcomp.p;
+    B.systime = (record_times && d_systime.p) ? d_systime.p : nullptr;
     B.gridcache = d_gridcache.p;
     B.gridcache_stride = ntotA;
     B.snap = (snap_on && d_snap.p) ? d_snap.p : nullptr;
@@ -464,7 +477,7 @@ void World::plan_classes() {
         const int N = info[s].natoms, M = info[s].nnode;
         int c = -1;
         for (int k = 0; k < 4; k++)
-            if (N <= blocks[k] && M <= blocks[k] && N + M <= 2 * blocks[k]) {
+            if (((class_mask >> k) & 1 || k == 3) && N <= blocks[k] && M <= blocks[k] && N + M <= 2 * blocks[k]) {
                 c = k;
                 break;
             }
@@ -477,9 +490,10 @@ void World::plan_classes() {
         cl[c].nmax = std::max(cl[c].nmax, N);
         cl[c].mmax = std::max(cl[c].mmax, M);
     }
-    if (too_big)
-        std::fprintf(stderr, "[libMMFFmulti_lib] %d systems exceed 256 atoms and are relaxed by the split path only\n",
-                     too_big);
+    n_unclassed = too_big;
+    if (too_big && verbosity > 0)
+        std::fprintf(stderr, "[libMMFFmulti_lib] %d systems exceed the 256-thread workgroup and are relaxed by the "
+                             "split kernels (plain FIRE)\n", too_big);
     int ncu = 256;
     {
         hipDeviceProp_t prop;
@@ -512,7 +526,9 @@ void World::plan_classes() {
         }
         auto cost = [&](int s) {
             const double N = info[s].natoms;
-            const double st = (nsteps.size() == (size_t)nSys && nsteps[s] > 0) ? nsteps[s] + 1.0 : 1000.0 + 20.0 * N;
+            const double st = (cost_feedback && nsteps.size() == (size_t)nSys && nsteps[s] > 0)
+                                  ? nsteps[s] + 1.0
+                                  : std::max(1.0, cost_c0 + cost_c1 * N);
             return st * N * N;
         };
         std::sort(C.order.begin(), C.order.end(), [&](int a, int b) {
@@ -559,8 +575,8 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
     last_chunks = 0;
     if (!dev_ready || nSys == 0) return -1;
     if (pc_mode == 2 && !learned_inkernel) return relax_learned(nstep_max, Fconv, restore_start);
-    if (pc_mode == 2 && (!d_mlpW.p || maxN > 256 || maxM > 256)) {
-        std::fprintf(stderr, "[libMMFFmulti_lib] learned gains need proposal weights and systems of <= 256 atoms\n");
+    if (pc_mode == 2 && !d_mlpW.p) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] learned gains need proposal weights\n");
         return -1;
     }
     if (!ring_ok) {
@@ -571,6 +587,7 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
     if (restore_start)
         hip_ok(hipMemcpyAsync(d_atoms.p, d_atoms0.p, sizeof(float4) * ntotV, hipMemcpyDeviceToDevice, stream),
                "restore");
+    if (record_times) d_systime.alloc((size_t)2 * nSys);
     fcd::BatchDev B = batch_dev();
     fcd::RunParams P = run_params(Fconv, nstep_max);
     std::vector<int> zeros(8, 0);
@@ -615,12 +632,48 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
         hipEventElapsedTime(&cm, starts[c], done[c]);
         classes[c].last_ms = cm;
     }
+    if (n_unclassed > 0 && !relax_unclassed(nstep_max, Fconv)) return -1;
     d_nsteps.download(nsteps, stream);
     d_status.download(status, stream);
     hip_ok(hipStreamSynchronize(stream), "relax results");
     int mx = 0;
     for (int s = 0; s < nSys; s++) mx = std::max(mx, nsteps[s]);
     return mx;
+}
+
+// Systems that fit no workgroup class of the persistent kernel (N or M above
+// 256, or N+M above 512) are relaxed after it by the split kernels, with plain
+// FIRE on the exact force: the other systems keep their final state and are
+// skipped.  Adds the device time to last_ms.
+bool World::relax_unclassed(int nstep_max, double Fconv) {
+    std::vector<int> only(nSys, 0);
+    for (int s = 0; s < nSys; s++) only[s] = (sys_class[s] < 0) ? 1 : 0;
+    d_only.upload(only, stream);
+    fcd::BatchDev B = batch_dev();
+    fcd::RunParams P = run_params(Fconv, nstep_max);
+    P.pc_mode = 0;
+    EventSet events;
+    hipEvent_t e0 = events.make(), e1 = events.make();
+    hipEventRecord(e0, stream);
+    if (!hip_ok(fc_launch_reset_subset(stream, &B, &P, d_only.p), "reset subset")) return false;
+    const int chunk = 50;
+    for (int step = 0; step <= nstep_max;) {
+        const int n = std::min(chunk, nstep_max + 1 - step);
+        for (int k = 0; k < n; k++, step++)
+            if (!hip_ok(fc_launch_split_step(stream, &B, &P, d_node_sys.p, ntotM, maxN, 1, step), "split step"))
+                return false;
+        d_status.download(status, stream);
+        if (!hip_ok(hipStreamSynchronize(stream), "split status")) return false;
+        bool any = false;
+        for (int s = 0; s < nSys && !any; s++) any = (status[s] == fcd::ST_RUNNING);
+        if (!any) break;
+    }
+    hipEventRecord(e1, stream);
+    hipStreamSynchronize(stream);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    last_ms += ms;
+    return true;
 }
 
 int World::relax_split(int nstep_max, double Fconv, bool restore_start) {

@@ -146,8 +146,19 @@ struct World {
     DevArr<float> d_Fmax, d_ebond;
     std::vector<SizeClass> classes;
     std::vector<int> qinfo, sys_class;
+    int n_unclassed = 0;  // systems too large for every class (relaxed by the split kernels)
+    bool record_times = false;  // per-system start/end wall clock of the persistent kernel
+    DevArr<long long> d_systime;
+    DevArr<int> d_only;
     bool hb_batch = false;
     bool autotune = true;
+    // cost estimate of a system for the queue order (longest first):
+    //   feedback: the step count of its previous relaxation when there is one,
+    //   else the model steps = cost_c0 + cost_c1 * N; model only: never uses
+    //   step counts of earlier relaxations of the same batch
+    bool cost_feedback = true;
+    int class_mask = 0xF;  // enabled workgroup widths: bit c = 64*(c+1) threads (256 always enabled)
+    double cost_c0 = 1000.0, cost_c1 = 20.0;
     // 0: device slots split between classes by estimated time, idle workgroups steal
     //    from smaller classes; 1: every class launched at full device width, no stealing
     int sched_mode = 1;
@@ -175,6 +186,7 @@ struct World {
     // device entry points
     int relax_persistent(int nstep_max, double Fconv, bool restore_start);
     int relax_split(int nstep_max, double Fconv, bool restore_start);
+    bool relax_unclassed(int nstep_max, double Fconv);
     bool eval_forces();
     bool eval_precond();
     bool sync();

@@ -70,6 +70,8 @@ def lib():
     L.set_opt.restype = None
     L.setSwitches_multi.argtypes = [c_int] * 9
     L.setSwitches_multi.restype = None
+    L.attach_surface.argtypes = [c_char_p, c_double]
+    L.attach_surface.restype = c_int
     _lib = L
     return L
 
@@ -180,6 +182,12 @@ def init_systems(systems):
     return n
 
 
+def attach_surface(surf="synth:NaCl", grid_step=0.1):
+    """Load (or create) the GridFF of `surf` and attach it to the current batch."""
+    if lib().attach_surface(surf.encode(), grid_step) != 0:
+        raise RuntimeError("attach_surface failed")
+
+
 def batch_sizes():
     out = np.zeros(9, np.int32)
     lib().get_batch_sizes(_p(out, c_int_p))
@@ -280,6 +288,42 @@ def relax(nstep_max, Fconv, path=0, restore_start=False, download=True):
 
 def last_ms():
     return lib().get_last_ms()
+
+
+def set_cost_model(feedback, c0=1000.0, c1=20.0):
+    L = lib()
+    L.set_cost_model.argtypes = [c_int, c_double, c_double]
+    L.set_cost_model.restype = c_int
+    if L.set_cost_model(int(feedback), c0, c1) != 0:
+        raise ValueError("set_cost_model: invalid coefficients")
+
+
+def set_class_mask(mask):
+    L = lib()
+    L.set_class_mask.argtypes = [c_int]
+    L.set_class_mask.restype = c_int
+    r = L.set_class_mask(int(mask))
+    if r < 0:
+        raise ValueError("set_class_mask: mask outside 0..15")
+    return r
+
+
+def set_record_times(on):
+    L = lib()
+    L.set_record_times.argtypes = [c_int]
+    L.set_record_times.restype = c_int
+    return L.set_record_times(int(on))
+
+
+def system_times():
+    """(nsys, 2) start/end of each system in the last recorded relaxation (us)."""
+    L = lib()
+    L.get_system_times.argtypes = [c_double_p]
+    L.get_system_times.restype = c_int
+    out = np.zeros((batch_sizes()["nsys"], 2))
+    if L.get_system_times(_p(out)) != 0:
+        raise RuntimeError("no system times recorded")
+    return out
 
 
 def class_info():

@@ -170,6 +170,10 @@ int init_systems(int nSys, const int* natoms, const int* nnode, const double* ap
  * surf_name (NULL = vacuum) places systems 3.0+-0.3 A above the substrate. */
 int init_workload(int kind, int nSys, long long seed0, int nmin, int nmax, double sigma, const char* surf_name,
                   double gridStep);
+/* Load (or create and save) the GridFF of surf_name and attach it to the
+ * current batch, as init(surf_name=...) does at initialisation (GridFF.h:
+ * 1787-1866 tryLoad_new).  0 on success, -1 on error. */
+int attach_surface(const char* surf_name, double gridStep);
 /* System i of init_workload (kinds 1,2) generated on the host only. */
 int gen_workload_arrays(int kind, int i, long long seed0, int nmin, int nmax, double sigma, const double* pos0,
                         const double* cell, int* natoms_out, int* nnode_out, double* apos, double* REQs, int* neighs,
@@ -180,6 +184,22 @@ int gen_workload_arrays(int kind, int i, long long seed0, int nmin, int nmax, do
 int relax(int nstepMax, double Fconv, int path, bool restore_start, bool bDownload);
 /* Device time of the last relax (HIP events), ms. */
 double get_last_ms(void);
+/* Queue order of the persistent kernel (longest first) by the estimated
+ * cost steps(N) * N^2 with steps(N) = c0 + c1*N; feedback != 0 uses instead a
+ * system's step count from the previous relaxation of the same batch when
+ * there is one.  0 on success, -1 on non-finite coefficients. */
+int set_cost_model(int feedback, double c0, double c1);
+/* Workgroup widths of the persistent kernel that systems may use: bit c
+ * enables 64*(c+1) threads (the 256-thread width is always enabled); a system
+ * goes to the narrowest enabled width that holds it.  Returns the previous
+ * mask, -1 if mask is outside 0..15. */
+int set_class_mask(int mask);
+/* Record each system's start and end (device wall clock) in later persistent
+ * relaxations (1) or not (0, default).  Returns the previous setting. */
+int set_record_times(int on);
+/* Start and end of every system in the last recorded relaxation, us after the
+ * earliest start, into out[2*s], out[2*s+1].  0 on success, -1 if none. */
+int get_system_times(double* out);
 /* Persistent-kernel size classes: {block, nsys, nmax, mmax, ngroups, lds}
  * per class and each launch's device time (ms) in the last relax. */
 int get_class_info(int* info6, double* ms, int maxc);

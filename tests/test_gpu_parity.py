@@ -338,6 +338,10 @@ def test_reference_style_init_run(nat, orc):
     mm.init(nSys_=16, xyz_name="synth:natoms=24,seed=0,sigma=0.1", nPBC=(0, 0, 0))
     mm.getBuffs()
     assert mm.gpu_atoms.shape == (16, mm.nvecs, 4) and mm.nvecs == mm.natoms + mm.nnode
+    # the reference's own view of gpu_neighs, (nSys, nvecs, 4), stays inside the padded mirror
+    ref_view = mm.getIBuff("gpu_neighs", (16, mm.nvecs, 4)).reshape(-1, 4)
+    assert np.array_equal(ref_view[:16 * mm.natoms], mm.gpu_neighs.reshape(-1, 4))
+    assert (ref_view[16 * mm.natoms:] == -1).all()
     mm.set_opt()
     outE = np.zeros(16)
     outF = np.zeros(16)
@@ -362,3 +366,195 @@ def test_sample_surf_matches_oracle(nat, orc):
     for z, e, fz in zip(zs, E, Fz):
         eo, fo = orc.grid_sample(g, np.array([0.3, -0.4, z]), plq)
         assert abs(e - eo) < 1e-4 + 1e-5 * abs(eo) and abs(fz - fo[2]) < 1e-4 + 1e-5 * abs(fo[2])
+
+
+# ------------------------------------------------------------------ large size classes
+def _fire_traj_precond(orc, m, fire, nsteps, grid, gstar, grigid):
+    """FP64 FIRE trajectory with the constant-gain preconditioner (oracle/precond.py)
+    applied to the exact atom forces before the per-vec clamp, following the
+    oracle's relax step (MolWorld_sp3::run_no_omp, MolWorld_sp3.h:2291-2474)."""
+    from oracle import precond as pc
+    dt_max, dt_min, damp_max, finc, fdec, falpha, minLastNeg, Flimit = fire
+    N, M = m["natoms"], m["nnode"]
+    comp = pc.components(N, m["neighs"])
+    gs = np.full(M, gstar)
+    gc = np.full(int(comp.max()) + 1, grigid)
+    pos = np.array(m["apos"], np.float64)
+    vel = np.zeros_like(pos)
+    dt, damping, lastNeg = dt_max, damp_max, 0
+    for _ in range(nsteps):
+        f = orc.eval(m, pos, grid=grid)[0]
+        f[:N] = pc.precondition(pos[:N], f[:N], m["neighs"], M, gs, gc, comp)
+        fl = np.sqrt((f ** 2).sum(1))
+        big = fl > Flimit
+        f[big] *= (Flimit / fl[big])[:, None]
+        vf, vv, ff = (vel * f).sum(), (vel * vel).sum(), (f * f).sum()
+        flen, vlen = np.sqrt(ff), np.sqrt(vv)
+        cos_vf = vf / (flen * vlen + 1e-32)
+        renorm = vlen / (flen + 1e-32)
+        if cos_vf < 0:
+            cv, cf = 0.0, 0.0
+            dt, lastNeg, damping = max(dt * fdec, dt_min), 0, damp_max
+        else:
+            cv, cf = 1 - damping, damping
+            if lastNeg > minLastNeg:
+                dt, damping = min(dt * finc, dt_max), damping * falpha
+            lastNeg += 1
+        cf *= renorm
+        dtf = float(np.float32(dt))
+        p = pos[N:]
+        f[N:] -= p * (p * f[N:]).sum(1, keepdims=True)
+        vel = vel * cv + f * (dtf + renorm * cf)
+        vel[N:] -= p * (p * vel[N:]).sum(1, keepdims=True)
+        pos = pos + vel * dtf
+        pos[N:] /= np.linalg.norm(pos[N:], axis=1, keepdims=True)
+    return pos
+
+
+@pytest.mark.parametrize("form", ["plain", "constant"])
+def test_large_class_trajectory_parity(nat, orc, form):
+    """Systems of 150, 200 and 256 atoms (the 192- and 256-thread workgroups,
+    whose windowed reaction rows wrap around the ring) follow the FP64
+    trajectory step by step for 40 FIRE steps on the GridFF, with the per-atom
+    window cache, plain and with the constant-gain preconditioner."""
+    P("case: large-class trajectory", form)
+    fire = tuple(FIRE_REF) + (10.0,)
+    nat.set_opt(*FIRE_REF)
+    nat.set_force_limit(10.0)
+    try:
+        # systems of fixed sizes (nmin = nmax = N) placed over the substrate
+        nat.init_workload(1, 1, 10001, 16, 16, 0.1, "synth:NaCl", 0.1)
+        g = nat.grid_info()
+        mols = [nat.gen_workload_system(1, k, 10101, n, n, 0.1, g) for k, n in enumerate((150, 200, 256))]
+        nat.init_systems(mols)
+        _attach_grid(nat)
+        if form == "constant":
+            nat.set_precond(1, 2.0, 63.0)
+        nat.relax(40, 1e-12, 0, True, True)
+        B = nat.Batch()
+        assert (B.nsteps == 40).all()
+        assert {int(c["block"]) for c in nat.class_info()} == {192, 256}
+        for s, m in enumerate(mols):
+            if form == "plain":
+                _, _, _, traj = orc.relax(m, fire, 1e-12, 40, 0, grid=g, ntraj=40)
+                ref = traj[-1]
+            else:
+                ref = _fire_traj_precond(orc, m, fire, 40, g, 2.0, 63.0)
+            d = np.abs(B.positions(s) - ref).max()
+            assert d < 2e-4, (s, m["natoms"], d)
+    finally:
+        nat.set_precond(0)
+        nat.set_force_limit(0.0)
+
+
+def _attach_grid(nat):
+    """Attach the synthetic NaCl GridFF to the current batch (init_systems
+    batches start without a substrate)."""
+    nat.attach_surface("synth:NaCl", 0.1)
+
+
+def test_grid_z_clamp_edge_on_device(nat, orc):
+    """Atoms above the top of the GridFF (iz clamped to nz-5 while tz keeps the
+    unclamped index, Bspline.h:759) on the persistent kernel: the first step
+    gathers the clamped window, the later ones read it from the window cache;
+    the 20-step trajectory follows the FP64 oracle."""
+    P("case: z-clamp edge")
+    fire = tuple(FIRE_REF) + (10.0,)
+    nat.set_opt(*FIRE_REF)
+    nat.set_force_limit(10.0)
+    try:
+        nat.init_workload(1, 1, 10001, 40, 40, 0.1, "synth:NaCl", 0.1)
+        g = nat.grid_info()
+        top = g["pos0"][2] + g["n"][2] * g["step"][2]
+        mols = []
+        for k, n in enumerate((24, 40)):
+            m = nat.gen_workload_system(1, k, 10201, n, n, 0.1, g)
+            a = np.array(m["apos"])
+            # centre the molecule 0.3 A below the top plane: its atoms straddle z = nz-5 and beyond
+            a[:n, 2] += (top - 0.3) - a[:n, 2].mean()
+            m["apos"] = a
+            mols.append(m)
+        nat.init_systems(mols)
+        _attach_grid(nat)
+        nat.relax(20, 1e-12, 0, True, True)
+        B = nat.Batch()
+        for s, m in enumerate(mols):
+            N = m["natoms"]
+            uz = (np.array(m["apos"])[:N, 2] - g["pos0"][2]) / g["step"][2]
+            assert (uz >= g["n"][2] - 4).any(), "no atom in the clamped band"
+            _, _, _, traj = orc.relax(m, fire, 1e-12, 20, 0, grid=g, ntraj=20)
+            d = np.abs(B.positions(s) - traj[-1]).max()
+            assert d < 2e-4, (s, d)
+    finally:
+        nat.set_force_limit(0.0)
+
+
+def test_sample_surf_z_edges(nat, orc):
+    """Device B-spline samples below iz=3 and above iz=nz-5 (the clamped bands)."""
+    P("case: sampleSurf z edges")
+    from firecore_amd import MMFF_multi as mm
+    import oracle
+    g0 = None
+    nat.init_workload(1, 1, 10001, 16, 16, 0.1, "synth:NaCl", 0.1)
+    g0 = nat.grid_info()
+    H = g0["n"][2] * g0["step"][2]
+    zs = np.concatenate([np.linspace(-0.4, 0.45, 9), np.linspace(H - 0.7, H + 0.5, 9)])
+    E, Fz = mm.sampleSurf("synth:NaCl", zs, kind=12, atyp=0, Q=0.1, K=1.5, pos0=(0.3, -0.4, 0.0))
+    g = nat.grid_info()
+    plq = oracle.req2plq(np.array([1.9255, np.sqrt(0.00455323095), 0.1, 0.0]), 1.5)
+    for z, e, fz in zip(zs, E, Fz):
+        eo, fo = orc.grid_sample(g, np.array([0.3, -0.4, z]), plq)
+        assert abs(e - eo) < 1e-4 + 2e-6 * abs(eo) and abs(fz - fo[2]) < 1e-4 + 2e-6 * abs(fo[2]), (z, e, eo, fz, fo)
+
+
+def test_oversize_system_relaxed_by_split_fallback(nat, orc):
+    """A system of ~300 atoms fits no workgroup class of the persistent kernel;
+    in a mixed batch it is relaxed by the split kernels after the persistent
+    launch, and its result (status, Fmax, forces, energy) is checked against
+    the FP64 oracle like every other system's."""
+    P("case: oversize system")
+    nat.set_opt(*FIRE_MULTI)
+    nat.set_force_limit(10.0)
+    try:
+        mols = [nat.gen_molecule(n, s, 0.1, s + 50) for n, s in [(24, 31), (300, 32), (80, 33)]]
+        nat.init_systems(mols)
+        nat.relax(20000, 1e-3, 0, True, True)
+        B = nat.Batch()
+        assert (B.status == nat.ST_CONVERGED).all(), B.status
+        assert 0 < B.nsteps[1] < 20000
+        for s, m in enumerate(mols):
+            fo, _, _, Eo = orc.eval(m, B.positions(s))
+            assert fmax(fo) < 1.01e-3, (s, fmax(fo))
+            assert abs(B.Fmax[s] - fmax(fo)) < 1e-4, (s, B.Fmax[s], fmax(fo))
+            assert abs(B.E[s] - Eo) < 1e-4 + 1e-6 * abs(Eo), (s, B.E[s], Eo)
+            assert np.abs(B.forces(s) - fo).max() < 1e-4, s
+    finally:
+        nat.set_force_limit(0.0)
+
+
+# ------------------------------------------------------------------ screening workload (molecule pairs)
+def test_pair_systems_forces(nat, orc):
+    """Molecule-pair systems of the screening workload (two molecules of 8-64
+    atoms each, random relative orientation, 4-8 A apart, over the GridFF):
+    forces and energies, including every intermolecular LJ+Coulomb pair, on
+    both kernel paths (split evaluation and a zero-step persistent pass)."""
+    P("case: molecule-pair forces")
+    nat.set_force_limit(0.0)
+    nat.init_workload(2, 16, 20001, 8, 64, 0.1, "synth:NaCl", 0.1)
+    g = nat.grid_info()
+    nat.eval_forces()
+    B = nat.Batch()
+    ncomp = [len(set(nat.iview("gpu_comp", B.sizes["ntotA"])[B.info[s, 2]:B.info[s, 2] + B.info[s, 0]]))
+             for s in range(B.nsys)]
+    assert all(c == 2 for c in ncomp), ncomp
+    check_forces(nat, orc, B, grid=g)
+    # the intermolecular pairs matter at this separation
+    m = nat.gen_workload_system(2, 0, 20001, 8, 64, 0.1, g)
+    comp = nat.iview("gpu_comp", B.sizes["ntotA"])[:m["natoms"]]
+    far = np.array(m["apos"])
+    far[:m["natoms"]][comp == 1] += np.array([0.0, 0.0, 500.0])
+    f_near = orc.eval(m, grid=None)[0]
+    f_far = orc.eval(m, far, grid=None)[0]
+    assert np.abs(f_near - f_far).max() > 1e-3
+    nat.relax(0, 1e-12, 0, True, True)
+    check_forces(nat, orc, nat.Batch(), grid=g)
