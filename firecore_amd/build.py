@@ -20,7 +20,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wno-unused-command-line-argument",
 HIP_FLAGS = ["-x", "hip", "--offload-arch=" + ARCH, "-munsafe-fp-atomics", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
-HIP_SOURCES = ["relax_persistent.hip", "split_kernels.hip", "sample_kernels.hip", "proposal_train.hip"]
+HIP_SOURCES = ["relax_persistent.hip", "split_kernels.hip", "sample_kernels.hip", "proposal_train.hip",
+               "gridff_gen.hip"]
 CPP_SOURCES = ["molgen.cpp", "topology.cpp", "gridff_host.cpp", "world.cpp", "train.cpp", "capi.cpp"]
 
 

@@ -22,6 +22,7 @@ extern "C" hipError_t fc_sample_angles(int kind, float k, float ang0, float r1, 
                                        float* E, float* F);
 extern "C" hipError_t fc_sample_grid(const fcd::GridDev* g, int n, const float4* pos, float4 plq, float4* out);
 extern "C" hipError_t fc_sample_pair(int n, const float* r, float4 REQij, float2* out);
+extern "C" hipError_t fc_gridff_nodes_device(const fc::Surface* s, const fc::GridFFHost* g, double* V);
 
 using fcw::World;
 using fcw::world;
@@ -94,7 +95,13 @@ bool setup_surface(const char* surf_name, double gridStep, fc::GridFFHost& g) {
     }
     const int nx = (int)std::lround(ax / gridStep), ny = (int)std::lround(by / gridStep),
               nz = (int)std::lround(cz / gridStep);
-    fc::make_gridff_bspline(S, nx, ny, nz, cz / nz, ztop, 1.5, 1.0, 4, 16, g);
+    // node potentials on the device (FP64), exact B-spline fit on the host
+    World& W = world();
+    if (!W.ensure_device()) return false;
+    fc::gridff_geometry(S, nx, ny, nz, cz / nz, ztop, 1.5, g);
+    std::vector<double> V((size_t)nx * ny * nz * 3);
+    if (!fcw::hip_ok(fc_gridff_nodes_device(&S, &g, V.data()), "GridFF node potentials")) return false;
+    fc::gridff_fit(g, V, 16);
     mkdirs(dir);
     std::vector<int> sh = {g.nx, g.ny, g.nz, 3};
     if (!fc::npy_save_f64(fn, sh, g.coefs.data()))
@@ -203,6 +210,8 @@ void* init(int nSys, char* xyz_name, char* surf_name, char* smile_name, bool bMM
     }
     std::vector<fc::MolSys> ms(nSys, tmpl);
     for (int i = 0; i < nSys; i++) fc::perturb_positions(ms[i], (uint64_t)(seed + 1 + i), sigma);
+    W.templ = tmpl;
+    W.has_templ = true;
     W.bMMFF = bMMFF;
     W.bGridFF = false;
     if (!W.set_systems(std::move(ms))) return nullptr;
@@ -305,9 +314,10 @@ void upload_pop(const char* fname) {
 }
 void scan(int nconf, double* poss, double* rots, double* Es, double* aforces, double* aposs, bool omp, bool bRelax,
           int niter_max, double dt, double Fconv, double Flim) {
-    (void)nconf; (void)poss; (void)rots; (void)Es; (void)aforces; (void)aposs; (void)omp; (void)bRelax;
-    (void)niter_max; (void)dt; (void)Fconv; (void)Flim;
-    unsupported("scan");
+    (void)omp;
+    (void)dt;
+    if (!world().scan(nconf, poss, rots, Es, aforces, aposs, bRelax, niter_max, Fconv, Flim))
+        std::fprintf(stderr, "[libMMFFmulti_lib] scan failed (no batch initialised, or a device error)\n");
 }
 void setSwitches_multi(int CheckInvariants, int PBC, int NonBonded, int MMFF, int Angles, int PiSigma, int PiPiI,
                        int dovdW, int bSaveToDatabase) {
@@ -683,11 +693,21 @@ int init_systems(int nSys, const int* natoms, const int* nnode, const double* ap
         m.Kpp.assign(Kpp + in * 4, Kpp + (in + M) * 4);
         if (elem) m.elem.assign(elem + ia, elem + ia + N);
         else m.elem.assign(N, 0);
+        // pi-vectors not given (zero rows): initial ones from the geometry, as
+        // MMFFsp3_loc::initPi + flipPis(1,1,1) do after building a molecule
+        bool given = true;
+        for (int k = 0; k < M && given; k++) {
+            const double* p = &m.apos[(N + k) * 3];
+            if (p[0] * p[0] + p[1] * p[1] + p[2] * p[2] < 1e-12) given = false;
+        }
+        if (!given) fc::init_pi(N, M, m.apos.data(), m.neighs.data(), m.Ksp.data(), m.Kpp.data(), true,
+                                m.apos.data() + (size_t)N * 3);
         ia += N;
         in += M;
         iv += V;
     }
     W.bGridFF = false;  // a new batch has no substrate until one is loaded
+    W.has_templ = false;
     if (!W.set_systems(std::move(ms))) return -1;
     if (!finish_init(W)) return -1;
     return W.ntotA;
@@ -730,10 +750,20 @@ int init_workload(int kind, int nSys, long long seed0, int nmin, int nmax, doubl
     }
     if (fail) return -1;
     W.bGridFF = false;
+    W.has_templ = false;
     if (!W.set_systems(std::move(ms))) return -1;
     if (surf) W.set_grid(g);
     if (!finish_init(W)) return -1;
     return W.ntotA;
+}
+
+int init_pi_vectors(int natoms, int nnode, const double* apos, const int* neighs, const double* Ksp,
+                    const double* Kpp, int flip, double* pipos) {
+    if (natoms <= 0 || nnode < 0 || nnode > natoms || !apos || !neighs || !Ksp || !Kpp || !pipos) return -1;
+    for (int i = 0; i < natoms * 4; i++)
+        if (neighs[i] >= natoms) return -1;
+    fc::init_pi(natoms, nnode, apos, neighs, Ksp, Kpp, flip != 0, pipos);
+    return 0;
 }
 
 int attach_surface(const char* surf_name, double gridStep) {
@@ -1105,11 +1135,61 @@ int make_exclusions(int natoms, int nnode, const int* neighs, int subtract13, un
     return 0;
 }
 // NaCl slab grid (direct sums + B-spline fit) without touching the device.
+// GridFF node potentials (Pauli, London, Coulomb) of a substrate at npts points
+// (host FP64 evaluation of the generator's definition).
+int gridff_point_values(int natoms, const double* apos, const double* REQs, const double* lvec9, double alpha,
+                        int npts, const double* pts, double* out) {
+    if (natoms <= 0 || npts < 0) return -1;
+    fc::Surface S;
+    S.apos.assign(apos, apos + natoms * 3);
+    S.REQs.assign(REQs, REQs + natoms * 4);
+    std::memcpy(S.lvec, lvec9, sizeof(S.lvec));
+    int nmx, nmy, ncx, ncy;
+    fc::gridff_images(S, nmx, nmy, ncx, ncy);
+    for (int i = 0; i < npts; i++) fc::gridff_point(S, pts + i * 3, alpha, nmx, nmy, ncx, ncy, out + i * 3);
+    return 0;
+}
+
+// Node potentials of the synthetic NaCl(001) GridFF grid (origin at the top
+// atomic plane, as make_nacl_gridff) computed by the device kernel (device=1)
+// or the host (device=0), (nx,ny,nz,3) z fastest.
+int gridff_nacl_nodes(int nx, int ny, int nz, double dz, int device, double* out) {
+    fc::Surface S;
+    fc::make_nacl_slab(5.64, 2, 0.9, S);
+    fc::GridFFHost g;
+    fc::gridff_geometry(S, nx, ny, nz, dz, 0.0, 1.5, g);
+    if (device) {
+        if (!world().ensure_device()) return -1;
+        return fcw::hip_ok(fc_gridff_nodes_device(&S, &g, out), "GridFF node potentials") ? 0 : -1;
+    }
+    std::vector<double> V;
+    fc::gridff_nodes_host(S, g, 16, V);
+    std::memcpy(out, V.data(), V.size() * sizeof(double));
+    return 0;
+}
+
+// Exact cubic B-spline coefficients of node values V (nx,ny,nz,3), periodic in
+// x,y and open in z, as used for every GridFF this library builds.
+int gridff_fit_values(int nx, int ny, int nz, const double* V, double* coefs) {
+    if (nx < 4 || ny < 4 || nz < 4) return -1;
+    fc::GridFFHost g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    std::vector<double> v(V, V + (size_t)nx * ny * nz * 3);
+    fc::gridff_fit(g, v, 8);
+    std::memcpy(coefs, g.coefs.data(), g.coefs.size() * sizeof(double));
+    return 0;
+}
+
 int make_nacl_gridff(int nx, int ny, int nz, double dz, int nthreads, double* coefs_out, double* geom_out7) {
     fc::Surface S;
     fc::make_nacl_slab(5.64, 2, 0.9, S);
     fc::GridFFHost g;
-    fc::make_gridff_bspline(S, nx, ny, nz, dz, 0.0, 1.5, 1.0, 4, nthreads, g);
+    fc::gridff_geometry(S, nx, ny, nz, dz, 0.0, 1.5, g);
+    std::vector<double> V;
+    fc::gridff_nodes_host(S, g, nthreads, V);
+    fc::gridff_fit(g, V, nthreads);
     if (coefs_out) std::memcpy(coefs_out, g.coefs.data(), g.coefs.size() * 8);
     for (int k = 0; k < 3; k++) {
         geom_out7[k] = g.pos0[k];

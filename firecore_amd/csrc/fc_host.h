@@ -76,6 +76,11 @@ bool make_workload_system(int kind, int i, long long seed0, int nmin, int nmax, 
 // makeBackNeighs (MMFFsp3_loc.h:1335-1358): bk[j] lists ia*4+k for every node
 // ia with neighs[ia][k]==j, filled first-empty.  Returns false on >4.
 bool make_back_neighs(int natoms, int nnode, const int* neighs, int* bkneighs);
+// Initial unit pi-vectors of the nodes from the geometry (MMFFsp3_loc::initPi,
+// MMFFsp3_loc.h:882-945, non-periodic): pipos (nnode x 3).  flip: then turn
+// every pi-vector into the half-space of (1,1,1) (flipPis, :1167-1172).
+void init_pi(int natoms, int nnode, const double* apos, const int* neighs, const double* Ksp, const double* Kpp,
+             bool flip, double* pipos);
 // Non-bonded pair weights in the bNonBondNeighs strategy:
 //   w(i,j) = [j not bonded to i] - (#nodes k whose angle loop subtracts {i,j})
 // Pairs with w==1 need nothing; every other pair (including i==j) is marked in
@@ -109,11 +114,14 @@ struct Surface {
 void make_nacl_slab(double a, int nlayers, double q, Surface& s);
 bool load_surface_xyz(const std::string& fname, Surface& s);
 bool save_surface_xyz(const std::string& fname, const Surface& s);
-// Direct lattice sums of the Pauli / London / damped-Coulomb channels on an
-// (nx,ny,nz) grid over one surface cell, then exact cubic B-spline
-// interpolation coefficients (periodic in x,y).
-void make_gridff_bspline(const Surface& s, int nx, int ny, int nz, double dz,
-                         double z0, double alphaMorse, double Rdamp, int nimg_coul,
-                         int nthreads, GridFFHost& g);
+// GridFF generation (gridff_host.cpp): node potentials (Pauli, London, bare
+// Coulomb) as the reference's GridFF::evalBsplineRef, then exact cubic
+// B-spline interpolation (periodic in x,y, open in z).
+constexpr double GRIDFF_R2MIN = 1e-4;  // (0.01 A)^2: lower bound of r^2 in the Coulomb node sums
+void gridff_images(const Surface& s, int& nmx, int& nmy, int& ncx, int& ncy);
+void gridff_point(const Surface& s, const double* p, double alpha, int nmx, int nmy, int ncx, int ncy, double* out);
+void gridff_geometry(const Surface& s, int nx, int ny, int nz, double dz, double z0, double alpha, GridFFHost& g);
+void gridff_nodes_host(const Surface& s, const GridFFHost& g, int nthreads, std::vector<double>& V);
+void gridff_fit(GridFFHost& g, std::vector<double>& V, int nthreads);
 
 }  // namespace fc

@@ -397,6 +397,25 @@ __device__ __forceinline__ float grid_sample_cached(const GridDev& g, V3 p, V3 p
 }
 
 // -------------------------------------------------------------------------
+// Positional restraint and bounding wall of one atom at p, the terms of the
+// reference's update kernel (updateAtomsMMFFf4, relax_multi.cl:997-1009):
+//   restraint (c.w > 0):  f += (c.xyz - p) * max(k.xyz, 0)
+//   wall on axis d with stiffness bc_d > 0:  f_d += (lo_d - p_d) bc_d below
+//   lo_d, (hi_d - p_d) bc_d above hi_d  (the reference enables z only; x and
+//   y follow the same rule when their stiffness is positive).
+__device__ __forceinline__ V3 restraint_force(V3 p, float4 c, float4 k) {
+    if (!(c.w > 0.f)) return zero3();
+    return mk((c.x - p.x) * fmaxf(k.x, 0.f), (c.y - p.y) * fmaxf(k.y, 0.f), (c.z - p.z) * fmaxf(k.z, 0.f));
+}
+__device__ __forceinline__ float wall1(float x, float lo, float hi, float k) {
+    if (!(k > 0.f)) return 0.f;
+    return (x < lo) ? (lo - x) * k : ((x > hi) ? (hi - x) * k : 0.f);
+}
+__device__ __forceinline__ V3 wall_force(V3 p, float4 lo, float4 hi, float4 k) {
+    return mk(wall1(p.x, lo.x, hi.x, k.x), wall1(p.y, lo.y, hi.y, k.y), wall1(p.z, lo.z, hi.z, k.z));
+}
+
+// -------------------------------------------------------------------------
 // FIRE parameters (DynamicOpt.h:47-95 defaults; set_opt, MMFFmulti_lib.cpp:141)
 struct FireParams {
     float dt_max, dt_min, damp_max, finc, fdec, falpha, cv_kill;

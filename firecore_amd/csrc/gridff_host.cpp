@@ -1,14 +1,23 @@
-// GridFF host side: the NaCl(001) substrate, direct lattice sums of the
-// Pauli / London / Coulomb channels, exact cubic B-spline coefficients and
-// the Bspline_PLQd.npy cache file.
+// GridFF host side: the NaCl(001) substrate, the node potentials of the
+// Pauli / London / Coulomb channels, exact cubic B-spline coefficients and the
+// Bspline_PLQd.npy cache file.
 //
-// Channel definitions follow GridFF::evalGridFFPoint (GridFF.h:542-580):
-//   Pauli   = sum_a  E_a exp(-2 alpha (r - R0_a))
-//   London  = sum_a -2 E_a exp(-alpha (r - R0_a))
-//   Coulomb = sum_a  k q_a / sqrt(r^2 + Rdamp^2)
-// with E_a = REQ_a.y (sqrt of the well depth, MMFFparams.h:129).  An atom with
-// PLQ = (e^2 E, e E, Q), e = exp(alpha R0) (REQ2PLQ_d, Forces.h:611-619) then
-// feels E = PLQ . (Pauli, London, Coulomb).
+// Node potentials follow the reference's B-spline grid generation
+// (GridFF::evalBsplineRef, GridFF.h:848-890):
+//   Pauli   = sum_a sum_img  E_a exp(-2 alpha (r - R0_a))   evalGridFFPoint_Mors
+//   London  = sum_a sum_img -2 E_a exp(-alpha (r - R0_a))   (GridFF.h:582-611)
+//   Coulomb = sum_a sum_img  k q_a / r                      evalGridFFPoint_Coul
+//                                                           (GridFF.h:614-635)
+// with E_a = REQ_a.y (sqrt of the well depth), Morse images within 20 A of the
+// cell (autoNPBC(cell, nPBC, 20), GridFF.h:43-48, initGridFF :1564) and
+// Coulomb images -25..25 in x and y (nPBC_coul, GridFF.h:849).  The reference
+// adds 1e-32 to r^2 in the Coulomb term; a node that coincides with an ion
+// (the grid origin lies in the top atomic plane) would then hold ~1e17 eV and
+// spoil the interpolation far above the surface, so here r^2 is bounded below
+// by (0.01 A)^2 -- a change confined to those nodes, whose influence on the
+// interpolating spline decays by (2-sqrt 3)^k over k nodes.
+// An atom with PLQ = (e^2 E, e E, Q), e = exp(alpha R0) (REQ2PLQ_d,
+// Forces.h:611-619) then feels E = PLQ . (Pauli, London, Coulomb).
 // The file layout is the reference's: shape (nx,ny,nz,3) float64, C order, z
 // fastest (GridFF::pack_Bspline_d, GridFF.h:1203-1217; tryLoad_new :1787-1866).
 #include "fc_host.h"
@@ -216,19 +225,52 @@ static void solve_open_141(int n, const double* v, int stride, double* c) {
     c[0] = v[0];
 }
 
-template <class F>
-static void parallel_for(int n, int nthreads, F f) {
-    nthreads = std::max(1, std::min(nthreads, n));
-    std::vector<std::thread> th;
-    for (int t = 0; t < nthreads; t++)
-        th.emplace_back([=]() {
-            for (int i = t; i < n; i += nthreads) f(i);
-        });
-    for (auto& x : th) x.join();
+// Images of the node sums: Morse within 20 A (autoNPBC), Coulomb -25..25.
+void gridff_images(const Surface& s, int& nmx, int& nmy, int& ncx, int& ncy) {
+    const double la = std::sqrt(s.lvec[0] * s.lvec[0] + s.lvec[1] * s.lvec[1] + s.lvec[2] * s.lvec[2]);
+    const double lb = std::sqrt(s.lvec[3] * s.lvec[3] + s.lvec[4] * s.lvec[4] + s.lvec[5] * s.lvec[5]);
+    nmx = (int)(20.0 / la);
+    nmy = (int)(20.0 / lb);
+    ncx = 25;
+    ncy = 25;
 }
 
-void make_gridff_bspline(const Surface& s, int nx, int ny, int nz, double dz, double z0, double alpha,
-                         double Rdamp, int nimg_coul, int nthreads, GridFFHost& g) {
+// Node potentials (P, L, Q) of one point, FP64, in the reference's image order
+// (atoms outer, images inner; GridFF.h:591-611, 620-634).
+void gridff_point(const Surface& s, const double* p, double alpha, int nmx, int nmy, int ncx, int ncy, double* out) {
+    const int na = s.natoms();
+    const double ax = s.lvec[0], ay = s.lvec[1], bx = s.lvec[3], by = s.lvec[4];
+    double P = 0, L = 0, Q = 0;
+    for (int a = 0; a < na; a++) {
+        const double* A = &s.apos[a * 3];
+        const double R0 = s.REQs[a * 4 + 0], E = s.REQs[a * 4 + 1], q = s.REQs[a * 4 + 2];
+        for (int jy = -nmy; jy <= nmy; jy++)
+            for (int jx = -nmx; jx <= nmx; jx++) {
+                const double dx = p[0] - A[0] + jx * ax + jy * bx, dy = p[1] - A[1] + jx * ay + jy * by,
+                             dz = p[2] - A[2];
+                const double r = std::sqrt(dx * dx + dy * dy + dz * dz + 1e-32);
+                const double e = std::exp(-alpha * (r - R0));
+                const double eM = e * E;
+                P += eM * e;
+                L += eM * -2.0;
+            }
+        for (int jy = -ncy; jy <= ncy; jy++)
+            for (int jx = -ncx; jx <= ncx; jx++) {
+                const double dx = p[0] - A[0] + jx * ax + jy * bx, dy = p[1] - A[1] + jx * ay + jy * by,
+                             dz = p[2] - A[2];
+                const double r2 = std::max(dx * dx + dy * dy + dz * dz, GRIDFF_R2MIN);
+                Q += COULOMB_K * q / std::sqrt(r2);
+            }
+    }
+    out[0] = P;
+    out[1] = L;
+    out[2] = Q;
+}
+
+template <class F>
+static void parallel_for(int n, int nthreads, F f);
+
+void gridff_geometry(const Surface& s, int nx, int ny, int nz, double dz, double z0, double alpha, GridFFHost& g) {
     g.nx = nx;
     g.ny = ny;
     g.nz = nz;
@@ -242,42 +284,29 @@ void make_gridff_bspline(const Surface& s, int nx, int ny, int nz, double dz, do
     g.pos0[0] = -0.5 * ax;
     g.pos0[1] = -0.5 * by;
     g.pos0[2] = z0;
-    const int na = s.natoms();
-    const size_t np = (size_t)nx * ny * nz;
-    std::vector<double> V(np * 3);
-    const double R2d = Rdamp * Rdamp;
-    const int nimg_m = 2;
-    parallel_for(nx, nthreads, [&](int ix) {
-        for (int iy = 0; iy < ny; iy++) {
-            for (int iz = 0; iz < nz; iz++) {
-                double px = g.pos0[0] + ix * g.step[0];
-                double py = g.pos0[1] + iy * g.step[1];
-                double pz = g.pos0[2] + iz * g.step[2];
-                double P = 0, L = 0, Q = 0;
-                for (int a = 0; a < na; a++) {
-                    const double* A = &s.apos[a * 3];
-                    const double R0 = s.REQs[a * 4 + 0], E = s.REQs[a * 4 + 1], q = s.REQs[a * 4 + 2];
-                    for (int jx = -nimg_coul; jx <= nimg_coul; jx++)
-                        for (int jy = -nimg_coul; jy <= nimg_coul; jy++) {
-                            double dx = px - (A[0] + jx * ax), dy = py - (A[1] + jy * by), dzz = pz - A[2];
-                            double r2 = dx * dx + dy * dy + dzz * dzz;
-                            Q += COULOMB_K * q / std::sqrt(r2 + R2d);
-                            if (std::abs(jx) <= nimg_m && std::abs(jy) <= nimg_m) {
-                                double r = std::sqrt(r2);
-                                double e = std::exp(-alpha * (r - R0));
-                                P += E * e * e;
-                                L += -2.0 * E * e;
-                            }
-                        }
-                }
-                size_t i = ((size_t)ix * ny + iy) * nz + iz;
-                V[i * 3 + 0] = P;
-                V[i * 3 + 1] = L;
-                V[i * 3 + 2] = Q;
+}
+
+// Node potentials of the whole grid on the host: V (nx,ny,nz,3), z fastest.
+void gridff_nodes_host(const Surface& s, const GridFFHost& g, int nthreads, std::vector<double>& V) {
+    int nmx, nmy, ncx, ncy;
+    gridff_images(s, nmx, nmy, ncx, ncy);
+    V.assign((size_t)g.nx * g.ny * g.nz * 3, 0.0);
+    parallel_for(g.nx, nthreads, [&](int ix) {
+        for (int iy = 0; iy < g.ny; iy++)
+            for (int iz = 0; iz < g.nz; iz++) {
+                const double p[3] = {g.pos0[0] + ix * g.step[0], g.pos0[1] + iy * g.step[1],
+                                     g.pos0[2] + iz * g.step[2]};
+                gridff_point(s, p, g.alphaMorse, nmx, nmy, ncx, ncy, &V[(((size_t)ix * g.ny + iy) * g.nz + iz) * 3]);
             }
-        }
     });
-    // separable interpolation: z (open), then y and x (periodic), per channel
+}
+
+// Exact cubic B-spline interpolation of node values V (nx,ny,nz,3): open in z
+// (linear extrapolation at both ends), periodic in y and x; coefficients into
+// g.coefs in the same layout.
+void gridff_fit(GridFFHost& g, std::vector<double>& V, int nthreads) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const size_t np = (size_t)nx * ny * nz;
     g.coefs.assign(np * 3, 0.0);
     std::vector<double> T(np * 3);
     parallel_for(nx * ny, nthreads, [&](int ixy) {
@@ -298,6 +327,17 @@ void make_gridff_bspline(const Surface& s, int nx, int ny, int nz, double dz, do
                 solve_cyclic_141(nx, &V[base], ny * nz * 3, &g.coefs[base]);
             }
     });
+}
+
+template <class F>
+static void parallel_for(int n, int nthreads, F f) {
+    nthreads = std::max(1, std::min(nthreads, n));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++)
+        th.emplace_back([=]() {
+            for (int i = t; i < n; i += nthreads) f(i);
+        });
+    for (auto& x : th) x.join();
 }
 
 }  // namespace fc

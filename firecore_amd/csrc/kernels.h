@@ -66,6 +66,15 @@ struct BatchDev {
     // row r of atom a at gridcache[r * gridcache_stride + a] (coalesced over atoms)
     float4* gridcache;
     int gridcache_stride;
+    // positional restraints and bounding walls (updateAtomsMMFFf4,
+    // relax_multi.cl:997-1009): per atom constr = (x, y, z, on if > 0) and
+    // constrK = per-axis stiffness (negative = 0); per system bbox = 3 float4
+    // (lower corner, upper corner, per-axis wall stiffness; <= 0 = no wall);
+    // sysflags bit 0: the system has a restrained atom, bit 1: it has a wall
+    const float4* constr;
+    const float4* constrK;
+    const float4* bbox;
+    const int* sysflags;
     // optional per-system wall-clock record (null = off): start and end of the
     // workgroup's relaxation of system s at systime[2s], systime[2s+1]
     long long* systime;

@@ -10,8 +10,10 @@
 //   star gain of node k        g_k = gstar  * sigmoid(o_k,0)
 //   molecule gain of comp. c   g_c = grigid * sigmoid(mean_{i in c} o_i,1)
 //
-// Dense layers run on the matrix cores: v_mfma_f32_16x16x16_bf16, BF16
-// weights and activations, FP32 accumulation; the 64->2 head and the
+// Dense layers run on the matrix cores, BF16 weights and activations, FP32
+// accumulation: the 128->64 layers with gfx950's v_mfma_f32_16x16x32_bf16
+// (K = 32 per instruction), the 16->64 input layer with
+// v_mfma_f32_16x16x16_bf16 (its K is 16); the 64->2 head and the
 // activations run on the vector ALUs in FP32.  Activations live in LDS as
 // bf16 rows of HS halfwords (64 + 4 padding, so that the 16 rows read by one
 // MFMA operand load fall into distinct banks).
@@ -25,6 +27,8 @@ constexpr int MLP_NIN = 16, MLP_NH = 64, MLP_NOUT = 2;
 constexpr int MLP_HS = 68;  // LDS row stride of the activations (bf16 elements)
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8s __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> bf16, round to nearest even (inputs are finite)
@@ -105,6 +109,69 @@ __device__ __forceinline__ bf16x4 mlp_a_frag(const unsigned short* h, const int4
     return a;
 }
 
+// The same for the 32-wide K chunks of v_mfma_f32_16x16x32_bf16 (lane l holds
+// row l%16, K values 8*(l/16)..+7 of the chunk): chunks 0,1 are the atom's own
+// row (64 columns), chunks 2,3 the bf16-rounded FP32 sum over its bonded
+// neighbours, element by element as in mlp_a_frag.
+__device__ __forceinline__ bf16x8 mlp_a_frag32(const unsigned short* h, const int4* nb, int n, int rb, int kc,
+                                               int lane) {
+    const int row = rb * 16 + (lane & 15);
+    const int kk = 8 * (lane >> 4);
+    bf16x8s a;
+    if (kc < 2) {
+        const int rr = min(row, n - 1);
+        const unsigned short* q = h + rr * MLP_HS + kc * 32 + kk;  // rows are 8-byte aligned
+        const uint2 u0 = *reinterpret_cast<const uint2*>(q);
+        const uint2 u1 = *reinterpret_cast<const uint2*>(q + 4);
+        a[0] = (short)(u0.x & 0xFFFFu);
+        a[1] = (short)(u0.x >> 16);
+        a[2] = (short)(u0.y & 0xFFFFu);
+        a[3] = (short)(u0.y >> 16);
+        a[4] = (short)(u1.x & 0xFFFFu);
+        a[5] = (short)(u1.x >> 16);
+        a[6] = (short)(u1.y & 0xFFFFu);
+        a[7] = (short)(u1.y >> 16);
+    } else {
+        const int4 n4 = (row < n) ? nb[row] : make_int4(-1, -1, -1, -1);
+        const int ns[4] = {n4.x, n4.y, n4.z, n4.w};
+        float sm[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            if (ns[t] < 0) continue;
+            const unsigned short* q = h + ns[t] * MLP_HS + (kc - 2) * 32 + kk;
+            const uint2 u0 = *reinterpret_cast<const uint2*>(q);
+            const uint2 u1 = *reinterpret_cast<const uint2*>(q + 4);
+            sm[0] += __uint_as_float(u0.x << 16);
+            sm[1] += __uint_as_float(u0.x & 0xFFFF0000u);
+            sm[2] += __uint_as_float(u0.y << 16);
+            sm[3] += __uint_as_float(u0.y & 0xFFFF0000u);
+            sm[4] += __uint_as_float(u1.x << 16);
+            sm[5] += __uint_as_float(u1.x & 0xFFFF0000u);
+            sm[6] += __uint_as_float(u1.y << 16);
+            sm[7] += __uint_as_float(u1.y & 0xFFFF0000u);
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) a[e] = (short)f2bf(sm[e]);
+    }
+    return __builtin_bit_cast(bf16x8, a);
+}
+
+// B operand (W^T) of column block cb, 32-wide K chunk kc: W row-major [64][128].
+__device__ __forceinline__ bf16x8 mlp_b_frag32(const unsigned short* W, int cb, int kc, int lane) {
+    const unsigned short* q = W + (cb * 16 + (lane & 15)) * 128 + kc * 32 + 8 * (lane >> 4);
+    const uint4 u = *reinterpret_cast<const uint4*>(q);  // 16-byte aligned: rows of 256 B
+    bf16x8s b;
+    b[0] = (short)(u.x & 0xFFFFu);
+    b[1] = (short)(u.x >> 16);
+    b[2] = (short)(u.y & 0xFFFFu);
+    b[3] = (short)(u.y >> 16);
+    b[4] = (short)(u.z & 0xFFFFu);
+    b[5] = (short)(u.z >> 16);
+    b[6] = (short)(u.w & 0xFFFFu);
+    b[7] = (short)(u.w >> 16);
+    return __builtin_bit_cast(bf16x8, b);
+}
+
 // B operand (W^T) of column block cb, K chunk kc: W row-major [64][KIN].
 template <int KIN>
 __device__ __forceinline__ bf16x4 mlp_b_frag(const unsigned short* W, int cb, int kc, int lane) {
@@ -136,11 +203,20 @@ __device__ __forceinline__ void mlp_layer(const unsigned short* h, const int4* n
         const int t = wid + q * NW;
         if (t < nrb * 4) {
             const int rb = t >> 2, cb = t & 3;
+            if constexpr (KIN == 128) {
 #pragma unroll
-            for (int kc = 0; kc < NK; kc++) {
-                const bf16x4 a = mlp_a_frag<KS>(h, nb, n, rb, kc, lane);
-                const bf16x4 b = mlp_b_frag<KIN>(W, cb, kc, lane);
-                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[q], 0, 0, 0);
+                for (int kc = 0; kc < 4; kc++) {
+                    const bf16x8 a = mlp_a_frag32(h, nb, n, rb, kc, lane);
+                    const bf16x8 b = mlp_b_frag32(W, cb, kc, lane);
+                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[q], 0, 0, 0);
+                }
+            } else {
+#pragma unroll
+                for (int kc = 0; kc < NK; kc++) {
+                    const bf16x4 a = mlp_a_frag<KS>(h, nb, n, rb, kc, lane);
+                    const bf16x4 b = mlp_b_frag<KIN>(W, cb, kc, lane);
+                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[q], 0, 0, 0);
+                }
             }
         }
     }
@@ -237,27 +313,48 @@ __device__ __forceinline__ void mlp_layer_rows(const unsigned short* hin, const 
     // (one memory latency); a wave then takes whole row blocks, builds their A
     // operand (own rows and neighbour sums) once and runs four independent
     // accumulation chains
-    bf16x4 bw[4][NK];
+    // K = 32 chunks for the 128-wide layers, K = 16 for the input layer
+    constexpr int NK32 = (KIN == 128) ? 4 : 1;
+    constexpr int NK16 = (KIN == 128) ? 1 : NK;
+    bf16x8 bw8[4][NK32];
+    bf16x4 bw[4][NK16];
 #pragma unroll
-    for (int cb = 0; cb < 4; cb++)
+    for (int cb = 0; cb < 4; cb++) {
+        if constexpr (KIN == 128) {
 #pragma unroll
-        for (int kc = 0; kc < NK; kc++) bw[cb][kc] = mlp_b_frag<KIN>(W, cb, kc, lane);
+            for (int kc = 0; kc < NK32; kc++) bw8[cb][kc] = mlp_b_frag32(W, cb, kc, lane);
+        } else {
+#pragma unroll
+            for (int kc = 0; kc < NK16; kc++) bw[cb][kc] = mlp_b_frag<KIN>(W, cb, kc, lane);
+        }
+    }
     float bc[4];
 #pragma unroll
     for (int cb = 0; cb < 4; cb++) bc[cb] = bias[cb * 16 + (lane & 15)];
 #pragma unroll 1
     for (int rb = wid; rb < nrb; rb += NW) {
-        bf16x4 a[NK];
-#pragma unroll
-        for (int kc = 0; kc < NK; kc++) a[kc] = mlp_a_frag<KS>(hin, nb, n, rb, kc, lane);
         f32x4 acc[4];
 #pragma unroll
         for (int cb = 0; cb < 4; cb++) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        if constexpr (KIN == 128) {
+            bf16x8 a[NK32];
 #pragma unroll
-        for (int kc = 0; kc < NK; kc++)
+            for (int kc = 0; kc < NK32; kc++) a[kc] = mlp_a_frag32(hin, nb, n, rb, kc, lane);
 #pragma unroll
-            for (int cb = 0; cb < 4; cb++)
-                acc[cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kc], bw[cb][kc], acc[cb], 0, 0, 0);
+            for (int kc = 0; kc < NK32; kc++)
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++)
+                    acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kc], bw8[cb][kc], acc[cb], 0, 0, 0);
+        } else {
+            bf16x4 a[NK16];
+#pragma unroll
+            for (int kc = 0; kc < NK16; kc++) a[kc] = mlp_a_frag<KS>(hin, nb, n, rb, kc, lane);
+#pragma unroll
+            for (int kc = 0; kc < NK16; kc++)
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++)
+                    acc[cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kc], bw[cb][kc], acc[cb], 0, 0, 0);
+        }
 #pragma unroll
         for (int cb = 0; cb < 4; cb++) {
             const int col = cb * 16 + (lane & 15);

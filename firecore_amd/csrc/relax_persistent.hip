@@ -146,6 +146,7 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
         if (B.systime && tid == 0 && P.chunk == 0) B.systime[2 * s] = (long long)wall_clock64();
         const SysInfo I = B.info[s];
         const int N = I.natoms, M = I.nnode, V = N + M;
+        const int sflags = B.sysflags ? B.sysflags[s] : 0;  // restraints (bit 0), walls (bit 1)
         const int W = (N + 31) >> 5;         // exclusion words per atom in memory
         const int Wh = ((N >> 1) >> 5) + 1;  // words of the offsets k <= N/2 the half ring reads
         // exclusion rows transposed to [word][atom] so that lanes read consecutive banks
@@ -378,6 +379,11 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                                         for (; e < nrow; e += N) f += xyz(row[e]);
                                     }
                                 if (v < M) f += xyz(s_fself[v]);
+                                if (sflags) {  // restraints and walls are part of the exact force
+                                    const V3 p = xyz(s_pos[v]);
+                                    if (sflags & 1) f += restraint_force(p, B.constr[I.a0 + v], B.constrK[I.a0 + v]);
+                                    if (sflags & 2) f += wall_force(p, B.bbox[3 * s], B.bbox[3 * s + 1], B.bbox[3 * s + 2]);
+                                }
                             } else {
                                 f = xyz(s_fself[M + (v - N)]);
                             }

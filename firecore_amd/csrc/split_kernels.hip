@@ -167,10 +167,16 @@ __global__ void __launch_bounds__(256) k_update(BatchDev B, RunParams P, int do_
     const float4* fng = B.fneigh;  // bkNeighs are absolute indices into fneigh
     float vf = 0.f, vv = 0.f, ff = 0.f, fm2 = 0.f;
     double e = 0.0;
+    const int sflags = B.sysflags ? B.sysflags[s] : 0;  // restraints (bit 0), walls (bit 1)
     for (int v = threadIdx.x; v < V; v += blockDim.x) {
         const float4 a = B.aforce[I.v0 + v];
         V3 f = xyz(a);
         if (v < N) e += B.enb[I.a0 + v];
+        if (v < N && sflags) {  // restraints and walls are part of the exact force
+            const V3 p = xyz(B.atoms[I.v0 + v]);
+            if (sflags & 1) f += restraint_force(p, B.constr[I.a0 + v], B.constrK[I.a0 + v]);
+            if (sflags & 2) f += wall_force(p, B.bbox[3 * s], B.bbox[3 * s + 1], B.bbox[3 * s + 2]);
+        }
         const int4 b = B.bkNeighs[I.v0 + v];
         if (b.x >= 0) { f += xyz(fng[b.x]);
         if (b.y >= 0) { f += xyz(fng[b.y]);

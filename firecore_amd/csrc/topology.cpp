@@ -2,6 +2,7 @@
 #include "fc_host.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 namespace fc {
@@ -121,6 +122,76 @@ int connected_components(int natoms, const int* neighs, int maxc, int* comp) {
         comp[i] = std::min(id[r], maxc - 1);
     }
     return std::min(nc, maxc);
+}
+
+}  // namespace fc
+
+namespace fc {
+
+// Pi-vectors of the nodes, MMFFsp3_loc::initPi (MMFFsp3_loc.h:882-945) for a
+// molecule without periodic images.  First pass: a node whose pi-sigma
+// stiffness Ksp is above Kmin for at least two neighbours takes the unit
+// normal of the plane through the first two such neighbours and the third
+// (or the node itself when there are only two), kept when its squared length
+// before normalisation exceeds r2min.  Second pass, over the nodes still
+// without a pi-vector in node order: the pi-vector of the neighbour with the
+// largest Kpp (strictly largest first found), if that Kpp exceeds Kmin, else
+// +z.  Then optionally flipPis(1,1,1) (:1167-1172).
+void init_pi(int natoms, int nnode, const double* apos, const int* neighs, const double* Ksp, const double* Kpp,
+             bool flip, double* pipos) {
+    (void)natoms;
+    const double Kmin = 1e-4, r2min = 1e-4;
+    for (int ia = 0; ia < nnode; ia++) {
+        const int* ng = neighs + ia * 4;
+        const double* ks = Ksp + ia * 4;
+        double pts[3][3];
+        int nfound = 0;
+        for (int j = 0; j < 4 && nfound < 3; j++) {
+            if (ks[j] > Kmin) {
+                for (int q = 0; q < 3; q++) pts[nfound][q] = apos[ng[j] * 3 + q];
+                nfound++;
+            }
+        }
+        double* out = pipos + ia * 3;
+        out[0] = out[1] = out[2] = 0.0;
+        if (nfound < 2) continue;
+        const double* p = (nfound == 2) ? apos + ia * 3 : pts[2];
+        const double u[3] = {pts[0][0] - p[0], pts[0][1] - p[1], pts[0][2] - p[2]};
+        const double v[3] = {pts[1][0] - p[0], pts[1][1] - p[1], pts[1][2] - p[2]};
+        const double c[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        const double r2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+        if (r2 > r2min) {
+            const double s = 1.0 / std::sqrt(r2);
+            for (int q = 0; q < 3; q++) out[q] = c[q] * s;
+        }
+    }
+    for (int ia = 0; ia < nnode; ia++) {
+        double* out = pipos + ia * 3;
+        if (out[0] * out[0] + out[1] * out[1] + out[2] * out[2] > 0.1) continue;
+        const int* ng = neighs + ia * 4;
+        const double* kp = Kpp + ia * 4;
+        int imax = -1;
+        double kmax = 0.0;
+        for (int i = 0; i < 4; i++)
+            if (kp[i] > kmax) {
+                imax = i;
+                kmax = kp[i];
+            }
+        if (kmax > Kmin && ng[imax] >= 0 && ng[imax] < nnode) {
+            const double* src = pipos + ng[imax] * 3;
+            for (int q = 0; q < 3; q++) out[q] = src[q];
+        } else {
+            out[0] = 0.0;
+            out[1] = 0.0;
+            out[2] = 1.0;
+        }
+    }
+    if (flip)
+        for (int ia = 0; ia < nnode; ia++) {
+            double* out = pipos + ia * 3;
+            if (out[0] + out[1] + out[2] < 0.0)
+                for (int q = 0; q < 3; q++) out[q] = -out[q];
+        }
 }
 
 }  // namespace fc

@@ -149,6 +149,13 @@ void World::build_packed() {
     REQs.assign(ntotA, f4(0, 0, 0, 0));
     PLQs.assign(ntotA, f4(0, 0, 0, 0));
     constr.assign(ntotA, f4(-1, -1, -1, -1));
+    constrK.assign(ntotA, f4(-1, -1, -1, -1));
+    bboxes.assign((size_t)nSys * 3, f4(0, 0, 0, 0));
+    for (int s = 0; s < nSys; s++) {
+        bboxes[s * 3] = f4(-1e8, -1e8, -1e8, 0);
+        bboxes[s * 3 + 1] = f4(1e8, 1e8, 1e8, 0);
+    }
+    sysflags.assign(nSys, 0);
     neighs.assign(ntotA, make_int4(-1, -1, -1, -1));
     neighCell.assign(ntotA, make_int4(-1, -1, -1, -1));
     bkNeighs.assign(ntotV, make_int4(-1, -1, -1, -1));
@@ -288,8 +295,29 @@ bool World::ensure_device() {
     return true;
 }
 
+// Per-system flags of the restraint terms: bit 0 = a restrained atom
+// (constr.w > 0), bit 1 = a wall (positive stiffness on some axis).
+void World::update_sysflags() {
+    sysflags.assign(nSys, 0);
+    for (int s = 0; s < nSys; s++) {
+        const fcd::SysInfo& I = info[s];
+        for (int i = 0; i < I.natoms; i++)
+            if (constr[I.a0 + i].w > 0.f) {
+                sysflags[s] |= 1;
+                break;
+            }
+        const float4 k = bboxes[s * 3 + 2];
+        if (k.x > 0.f || k.y > 0.f || k.z > 0.f) sysflags[s] |= 2;
+    }
+}
+
 bool World::upload_all(bool bParams, bool bForces, bool bVel) {
     if (!ensure_device()) return false;
+    update_sysflags();
+    d_constr.upload(constr, stream);
+    d_constrK.upload(constrK, stream);
+    d_bboxes.upload(bboxes, stream);
+    d_sysflags.upload(sysflags, stream);
     d_atoms.upload(atoms, stream);
     d_atoms0.upload(atoms, stream);
     if (bForces || !d_aforce.p) d_aforce.upload(aforces, stream);
@@ -409,6 +437,10 @@ fcd::BatchDev World::batch_dev() {
     B.gnode = d_gnode.p;
     B.gcomp = d_gcomp.p;
     B.systime = (record_times && d_systime.p) ? d_systime.p : nullptr;
+    B.constr = d_constr.p;
+    B.constrK = d_constrK.p;
+    B.bbox = d_bboxes.p;
+    B.sysflags = d_sysflags.p;
     B.gridcache = d_gridcache.p;
     B.gridcache_stride = ntotA;
     B.snap = (snap_on && d_snap.p) ? d_snap.p : nullptr;
@@ -641,6 +673,95 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
     return mx;
 }
 
+// Screening scan (MolWorld_sp3_multi::scan_relaxed, MolWorld_sp3_multi.h:
+// 1242-1296): configuration i is the template molecule moved by
+// setFromRef(pos_i, rot_i) (atoms rot^T a + pos, pi-vectors rot^T p,
+// MMFFsp3_loc.h:1543-1551) with atom 0 restrained to pos_i (constr.w = 1,
+// constrK = 1, as the reference sets for its picked atom).  The reference
+// relaxes nSystems configurations at a time and refills the slots that
+// converged from the host; here all nconf configurations form one batch and
+// the persistent kernel's work queue does the refill on the device: a
+// workgroup takes the next configuration as soon as its system retires.
+// Outputs per configuration: Es = force-field energy at the relaxed geometry
+// (no restraint energy, as the reference's eval_no_omp), aforces = the
+// force-field force on each atom (the restraint force removed), aposs = atom
+// positions.  bRelax = false evaluates the placed configurations without
+// relaxing them.  The batch that was loaded before is put back afterwards.
+bool World::scan(int nconf, const double* poss, const double* rots, double* Es, double* aforces, double* aposs,
+                 bool bRelax, int niter_max, double Fconv, double Flim) {
+    if (!dev_ready || nSys == 0 || nconf <= 0) return false;
+    // keep the current batch (device state included) to restore it afterwards
+    if (!download_all(true, true)) return false;
+    for (int s = 0; s < nSys; s++) unpack_system(s, true, true);
+    std::vector<fc::MolSys> saved = mols;
+    std::vector<float4> saved_constr = constr, saved_constrK = constrK, saved_bboxes = bboxes;
+    const bool saved_grid = bGridFF;
+    const fc::MolSys tmpl = has_templ ? templ : ffl;  // the template molecule of init
+    const int N = tmpl.natoms, M = tmpl.nnode;
+    std::vector<fc::MolSys> ms(nconf, tmpl);
+    for (int i = 0; i < nconf; i++) {
+        const double* p = poss ? poss + i * 3 : nullptr;
+        const double* R = rots ? rots + i * 9 : nullptr;
+        fc::MolSys& m = ms[i];
+        for (int v = 0; v < N + M; v++) {
+            const double* a = &tmpl.apos[v * 3];
+            double o[3] = {a[0], a[1], a[2]};
+            if (R)  // rot^T a  (Mat3::dot_to_T, rows of R in memory order)
+                for (int q = 0; q < 3; q++) o[q] = R[q] * a[0] + R[3 + q] * a[1] + R[6 + q] * a[2];
+            if (p && v < N)
+                for (int q = 0; q < 3; q++) o[q] += p[q];
+            for (int q = 0; q < 3; q++) m.apos[v * 3 + q] = o[q];
+        }
+        m.fapos.clear();
+        m.vapos.clear();
+    }
+    set_systems(std::move(ms));  // the substrate grid (if any) stays loaded
+    for (int i = 0; i < nconf; i++) {
+        const int a0 = info[i].a0;
+        const double* p = poss ? poss + i * 3 : nullptr;
+        constr[a0] = make_float4(p ? (float)p[0] : 0.f, p ? (float)p[1] : 0.f, p ? (float)p[2] : 0.f, 1.f);
+        constrK[a0] = make_float4(1.f, 1.f, 1.f, 1.f);
+    }
+    bool ok = upload_all(true, true, true);
+    const float Flimit_keep = Flimit;
+    if (Flim > 0) Flimit = (float)Flim;
+    if (ok) ok = relax_persistent(bRelax ? niter_max : 0, Fconv, false) >= 0;
+    Flimit = Flimit_keep;
+    if (ok) ok = download_all(true, true);
+    if (ok)
+        for (int i = 0; i < nconf; i++) {
+            const fcd::SysInfo& I = info[i];
+            if (Es) Es[i] = Esys[i];
+            for (int a = 0; a < N; a++) {
+                const float4 x = atoms[I.v0 + a];
+                float4 f = this->aforces[I.v0 + a];
+                if (a == 0) {  // remove the restraint force of atom 0
+                    const float4 c = constr[I.a0], k = constrK[I.a0];
+                    f.x -= (c.x - x.x) * std::max(k.x, 0.f);
+                    f.y -= (c.y - x.y) * std::max(k.y, 0.f);
+                    f.z -= (c.z - x.z) * std::max(k.z, 0.f);
+                }
+                if (aforces) {
+                    aforces[((size_t)i * N + a) * 3] = f.x;
+                    aforces[((size_t)i * N + a) * 3 + 1] = f.y;
+                    aforces[((size_t)i * N + a) * 3 + 2] = f.z;
+                }
+                if (aposs) {
+                    aposs[((size_t)i * N + a) * 3] = x.x;
+                    aposs[((size_t)i * N + a) * 3 + 1] = x.y;
+                    aposs[((size_t)i * N + a) * 3 + 2] = x.z;
+                }
+            }
+        }
+    // the previous batch back
+    set_systems(std::move(saved));
+    constr = saved_constr;
+    constrK = saved_constrK;
+    bboxes = saved_bboxes;
+    bGridFF = saved_grid;
+    return upload_all(true, true, true) && ok;
+}
+
 // Systems that fit no workgroup class of the persistent kernel (N or M above
 // 256, or N+M above 512) are relaxed after it by the split kernels, with plain
 // FIRE on the exact force: the other systems keep their final state and are
@@ -811,6 +932,8 @@ void World::register_buffers() {
     fbuffers["gpu_aforces"] = (float*)aforces.data();
     fbuffers["gpu_avel"] = (float*)avel.data();
     fbuffers["gpu_constr"] = (float*)constr.data();
+    fbuffers["gpu_constrK"] = (float*)constrK.data();
+    fbuffers["gpu_bboxes"] = (float*)bboxes.data();
     fbuffers["gpu_REQs"] = (float*)REQs.data();
     fbuffers["gpu_MMpars"] = (float*)MMpars.data();
     fbuffers["gpu_BLs"] = (float*)BLs.data();

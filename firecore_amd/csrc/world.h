@@ -104,6 +104,8 @@ struct World {
     // ---- systems (host, double) and the template system (system 0)
     std::vector<fc::MolSys> mols;
     fc::MolSys ffl;
+    fc::MolSys templ;         // the template molecule of init() (scan places copies of it)
+    bool has_templ = false;
     int nSys = 0, ntotA = 0, ntotM = 0, ntotV = 0, maxN = 0, maxM = 0;
     int nvecs_uniform = 0, natoms_uniform = 0, nnode_uniform = 0;  // >0 for uniform batches
     std::vector<fcd::SysInfo> info;
@@ -111,6 +113,11 @@ struct World {
     // ---- host mirrors in the reference's packed float layout
     std::vector<float4> pcforces;  // preconditioned forces of the last eval_precond
     std::vector<float4> atoms, aforces, avel, REQs, PLQs, MMpars, BLs, BKs, Ksp, Kpp, constr, lvecs, ilvecs, pbcshifts;
+    // restraints of the update kernel: constrK per atom (per-axis stiffness), bboxes
+    // per system (lower corner, upper corner, per-axis wall stiffness), flags per system
+    std::vector<float4> constrK, bboxes;
+    std::vector<int> sysflags;
+    void update_sysflags();
     std::vector<int4> neighs, neighCell, bkNeighs;
     std::vector<unsigned> excl, excl_rot;
     bool ring_ok = true;
@@ -141,7 +148,8 @@ struct World {
     DevArr<unsigned> d_excl, d_excl_rot;
     DevArr<fcd::PairCorrDev> d_corr;
     DevArr<fcd::SysInfo> d_info;
-    DevArr<int> d_node_sys, d_nsteps, d_status, d_order, d_queue, d_comp, d_ncomp, d_elem;
+    DevArr<int> d_node_sys, d_nsteps, d_status, d_order, d_queue, d_comp, d_ncomp, d_elem, d_sysflags;
+    DevArr<float4> d_constr, d_constrK, d_bboxes;
     DevArr<double> d_Esys, d_enb;
     DevArr<float> d_Fmax, d_ebond;
     std::vector<SizeClass> classes;
@@ -187,6 +195,10 @@ struct World {
     int relax_persistent(int nstep_max, double Fconv, bool restore_start);
     int relax_split(int nstep_max, double Fconv, bool restore_start);
     bool relax_unclassed(int nstep_max, double Fconv);
+    // screening caller (MolWorld_sp3_multi::scan_relaxed): every configuration
+    // a system of one batch, relaxed together; the previous batch is restored
+    bool scan(int nconf, const double* poss, const double* rots, double* Es, double* aforces, double* aposs,
+              bool bRelax, int niter_max, double Fconv, double Flim);
     bool eval_forces();
     bool eval_precond();
     bool sync();

@@ -139,6 +139,22 @@ def make_back_neighs(natoms, nnode, neighs):
     return r, bk
 
 
+def init_pi_vectors(apos, neighs, nnode, Ksp, Kpp, flip=True):
+    """Initial pi-vectors (nnode,3) of a molecule from its atom positions."""
+    apos = np.ascontiguousarray(apos, np.float64)
+    neighs = np.ascontiguousarray(neighs, np.int32)
+    natoms = len(neighs)
+    Ksp = np.ascontiguousarray(Ksp, np.float64)
+    Kpp = np.ascontiguousarray(Kpp, np.float64)
+    out = np.zeros((nnode, 3))
+    L = lib()
+    L.init_pi_vectors.argtypes = [c_int, c_int, c_double_p, c_int_p, c_double_p, c_double_p, c_int, c_double_p]
+    L.init_pi_vectors.restype = c_int
+    if L.init_pi_vectors(natoms, nnode, _p(apos), _p(neighs, c_int_p), _p(Ksp), _p(Kpp), int(flip), _p(out)) != 0:
+        raise ValueError("init_pi_vectors: bad input")
+    return out
+
+
 def make_exclusions(natoms, nnode, neighs, subtract13=True):
     neighs = np.ascontiguousarray(neighs, np.int32)
     W = (natoms + 31) // 32
@@ -250,6 +266,9 @@ class Batch:
         self.BKs = fview("gpu_BKs", M * 4).reshape(M, 4)
         self.Ksp = fview("gpu_Ksp", M * 4).reshape(M, 4)
         self.Kpp = fview("gpu_Kpp", M * 4).reshape(M, 4)
+        self.constr = fview("gpu_constr", A * 4).reshape(A, 4)
+        self.constrK = fview("gpu_constrK", A * 4).reshape(A, 4)
+        self.bboxes = fview("gpu_bboxes", ns * 12).reshape(ns, 3, 4)
         self.nsteps = iview("sys_nsteps", ns)
         self.status = iview("sys_status", ns)
         self.Fmax = fview("sys_Fmax", ns)
@@ -277,6 +296,14 @@ class Batch:
     def positions(self, s):
         N, M, a0, n0, v0 = (int(x) for x in self.info[s, :5])
         return self.atoms[v0:v0 + N + M, :3].astype(np.float64)
+
+
+def upload(bParams=True, bForces=False, bVel=False, blvec=True):
+    """Copy the host mirrors (positions, restraints, walls, parameters) to the device."""
+    L = lib()
+    L.upload.argtypes = [c_bool] * 4
+    L.upload.restype = None
+    L.upload(bParams, bForces, bVel, blvec)
 
 
 def relax(nstep_max, Fconv, path=0, restore_start=False, download=True):

@@ -18,6 +18,7 @@
 #define FIRECORE_AMD_MMFF_MULTI_H
 
 #include <stdbool.h>
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -96,7 +97,16 @@ void optimizeLattice_1d(double* dlvec, int n1, int n2, int initMode, double tol)
 void optimizeLattice_2d_multi(double* dlvec, int nstesp, int initMode, double tol);
 /* MMFFmulti_lib.cpp:282-288  not supported: no-op */
 void upload_pop(const char* fname);
-/* MMFFmulti_lib.cpp:305-314  not supported (screening refill): no-op */
+/* MMFFmulti_lib.cpp:305-314 -> MolWorld_sp3_multi::scan_relaxed
+ * (MolWorld_sp3_multi.h:1242-1296): configuration i = the template molecule
+ * of init moved by rots[i] (3x3, applied transposed, Mat3::dot_to_T) and
+ * poss[i], atom 0 restrained to poss[i] (stiffness 1); all nconf
+ * configurations are relaxed as one batch on the device (the persistent
+ * kernel's queue refills retired slots), FIRE to max|f| < Fconv, at most
+ * niter_max steps, per-vec force limit Flim.  Outputs (may be NULL): Es[i]
+ * force-field energy, aforces[i] (natoms x 3) force-field forces (restraint
+ * removed), aposs[i] (natoms x 3) positions.  bRelax = false: no relaxation.
+ * dt and omp are ignored.  The batch loaded before is restored afterwards. */
 void scan(int nconf, double* poss, double* rots, double* Es, double* aforces, double* aposs, bool omp, bool bRelax,
           int niter_max, double dt, double Fconv, double Flim);
 /* MMFFmulti_lib.cpp:316-329  tri-state switches (>0 on, <0 off, 0 keep);
@@ -170,6 +180,14 @@ int init_systems(int nSys, const int* natoms, const int* nnode, const double* ap
  * surf_name (NULL = vacuum) places systems 3.0+-0.3 A above the substrate. */
 int init_workload(int kind, int nSys, long long seed0, int nmin, int nmax, double sigma, const char* surf_name,
                   double gridStep);
+/* Initial unit pi-vectors of the nnode nodes (pipos, nnode x 3) from the atom
+ * positions (apos, natoms x 3), neighbour lists and the Ksp/Kpp stiffnesses:
+ * MMFFsp3_loc::initPi (MMFFsp3_loc.h:882-945) without periodic images, then
+ * with flip != 0 flipPis((1,1,1)) (:1167-1172).  init_systems applies it to
+ * every system whose pi-vector rows are all zero.  0 on success, -1 on bad
+ * input. */
+int init_pi_vectors(int natoms, int nnode, const double* apos, const int* neighs, const double* Ksp,
+                    const double* Kpp, int flip, double* pipos);
 /* Load (or create and save) the GridFF of surf_name and attach it to the
  * current batch, as init(surf_name=...) does at initialisation (GridFF.h:
  * 1787-1866 tryLoad_new).  0 on success, -1 on error. */
@@ -269,6 +287,19 @@ int make_back_neighs(int natoms, int nnode, const int* neighs, int* bkneighs);
 int make_exclusions(int natoms, int nnode, const int* neighs, int subtract13, unsigned* mask_out, int* W_out,
                     int* ncorr_out);
 int make_nacl_gridff(int nx, int ny, int nz, double dz, int nthreads, double* coefs_out, double* geom_out7);
+/* GridFF generation (the reference's GridFF::evalBsplineRef node potentials,
+ * GridFF.h:848-890: Pauli and London over the Morse images within 20 A, bare
+ * Coulomb over images -25..25, r^2 >= 1e-4 A^2; then exact cubic B-spline
+ * interpolation, periodic in x,y, open in z).  init()/init_workload compute the
+ * node potentials on the device and fit on the host.
+ *   gridff_point_values: (P, L, Q) of a substrate at npts points, host FP64
+ *   gridff_nacl_nodes:   node potentials of the synthetic NaCl(001) grid on the
+ *                        device (device=1) or the host (0)
+ *   gridff_fit_values:   interpolation coefficients of node values V */
+int gridff_point_values(int natoms, const double* apos, const double* REQs, const double* lvec9, double alpha,
+                        int npts, const double* pts, double* out);
+int gridff_nacl_nodes(int nx, int ny, int nz, double dz, int device, double* out);
+int gridff_fit_values(int nx, int ny, int nz, const double* V, double* coefs);
 
 #ifdef __cplusplus
 }

@@ -100,6 +100,19 @@ class Backend:
         n = self._fn("relax")(*args)
         return n, apos, E.value, traj[:ntraj]
 
+    def set_restraints(self, natoms=0, constr=None, constrK=None, bbox=None):
+        """Positional restraints / walls of the GPU update kernel for the
+        following eval/relax calls on systems of `natoms` atoms (0 clears).
+        Only the FP64 restatement has them (the reference CPU path has none)."""
+        f = self._fn("set_restraints")
+        if natoms <= 0:
+            f(ctypes.c_int(0), None, None, None)
+            return
+        c = np.ascontiguousarray(constr, np.float64)
+        k = np.ascontiguousarray(constrK, np.float64)
+        b = np.ascontiguousarray(np.zeros(9) if bbox is None else bbox, np.float64).reshape(9)
+        f(ctypes.c_int(natoms), _p(c), _p(k), _p(b))
+
     def make_back_neighs(self, natoms, nnode, neighs):
         neighs = np.ascontiguousarray(neighs, dtype=np.int32)
         bk = np.zeros((natoms, 4), np.int32)
@@ -133,6 +146,39 @@ class Backend:
         F = np.zeros(len(rs))
         self._fn("sample_pair")(ctypes.c_int(len(rs)), _p(rs), _p(REQij), _p(E), _p(F))
         return E, F
+
+
+def ref_init_pi(apos, neighs, nnode, Ksp, Kpp, flip=True):
+    """The reference's own MMFFsp3_loc::initPi (+ flipPis) through oracle/_ref,
+    or None where the reference build is absent."""
+    if not os.path.exists(REF_SO):
+        return None
+    lib = ctypes.CDLL(REF_SO)
+    apos = np.ascontiguousarray(apos, np.float64)
+    neighs = np.ascontiguousarray(neighs, np.int32)
+    out = np.zeros((nnode, 3))
+    lib.ref_init_pi(ctypes.c_int(len(neighs)), ctypes.c_int(nnode), _p(apos), _p(neighs, _ip),
+                    _p(np.ascontiguousarray(Ksp, np.float64)), _p(np.ascontiguousarray(Kpp, np.float64)),
+                    ctypes.c_int(int(flip)), _p(out))
+    return out
+
+
+def ref_gridff_points(apos, REQs, lvec, alpha, nm, nc, pts):
+    """The reference's own GridFF node potentials (Pauli, London, bare Coulomb)
+    through oracle/_ref at points pts (n,3); None where the build is absent.
+    nm, nc: (x, y) image ranges of the Morse and Coulomb sums."""
+    if not os.path.exists(REF_SO):
+        return None
+    lib = ctypes.CDLL(REF_SO)
+    apos = np.ascontiguousarray(apos, np.float64)
+    REQs = np.ascontiguousarray(REQs, np.float64)
+    lvec = np.ascontiguousarray(lvec, np.float64).reshape(9)
+    pts = np.ascontiguousarray(pts, np.float64)
+    out = np.zeros((len(pts), 3))
+    lib.ref_gridff_points(ctypes.c_int(len(apos)), _p(apos), _p(REQs), _p(lvec), ctypes.c_double(alpha),
+                          ctypes.c_int(nm[0]), ctypes.c_int(nm[1]), ctypes.c_int(nc[0]), ctypes.c_int(nc[1]),
+                          ctypes.c_int(len(pts)), _p(pts), _p(out))
+    return out
 
 
 def oracle():

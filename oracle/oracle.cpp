@@ -64,9 +64,22 @@ struct Sys {
     const double* gc = nullptr;
     int gn[3] = {0, 0, 0};
     double gp0[3], gstep[3], alpha = 1.5;
+    // positional restraints and walls of the GPU update kernel (optional)
+    const double* constr = nullptr;   // natoms x 4 (x, y, z, on if > 0)
+    const double* constrK = nullptr;  // natoms x 4 (per-axis stiffness; negative = 0)
+    double bbox[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // lower corner, upper corner, wall stiffness
     // work
     std::vector<double> f, fng, fngpi;
 };
+
+// Restraints for the following orc_eval / orc_relax calls (natoms = 0 clears);
+// the arrays are copied.
+struct Restraints {
+    int natoms = 0;
+    std::vector<double> constr, constrK;
+    double bbox[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+};
+Restraints g_restraints;
 
 // getLJQH, Forces.h:341-360.  dp = pj - pi, returns E; f = force on i.
 double pair_ljqh(V dp, double R, double E0, double Q, double H, V& f) {
@@ -359,6 +372,23 @@ double eval_sys(Sys& S) {
             if (bpi) fp += ld(&S.fngpi[j * 3]);
         }
         st(&S.f[ia * 3], ld(&S.f[ia * 3]) + fa);
+        // restraint and wall of the GPU update kernel, updateAtomsMMFFf4
+        // (relax_multi.cl:997-1009), as part of the atom force
+        if (S.constr && S.constr[ia * 4 + 3] > 0) {
+            const double* c = S.constr + ia * 4;
+            const double* k = S.constrK + ia * 4;
+            const V p = ld(&S.pos[ia * 3]);
+            const V fc{(c[0] - p.x) * std::max(k[0], 0.0), (c[1] - p.y) * std::max(k[1], 0.0),
+                       (c[2] - p.z) * std::max(k[2], 0.0)};
+            st(&S.f[ia * 3], ld(&S.f[ia * 3]) + fc);
+        }
+        for (int d = 0; d < 3; d++) {
+            const double kw = S.bbox[6 + d];
+            if (!(kw > 0)) continue;
+            const double x = S.pos[ia * 3 + d], lo = S.bbox[d], hi = S.bbox[3 + d];
+            if (x < lo) S.f[ia * 3 + d] += (lo - x) * kw;
+            else if (x > hi) S.f[ia * 3 + d] += (hi - x) * kw;
+        }
         if (bpi) {
             V f = ld(&S.f[(N + ia) * 3]) + fp;
             V p = ld(&S.pos[(N + ia) * 3]);
@@ -388,6 +418,11 @@ void setup(Sys& S, int natoms, int nnode, const double* apos, const double* REQs
     S.doPiSigma = sw[2];
     S.doAngles = sw[3];
     S.bNonBonded = sw[4];
+    if (g_restraints.natoms == natoms) {
+        S.constr = g_restraints.constr.data();
+        S.constrK = g_restraints.constrK.data();
+        for (int k = 0; k < 9; k++) S.bbox[k] = g_restraints.bbox[k];
+    }
     S.gc = gcoefs;
     if (gcoefs) {
         S.gn[0] = gnx;
@@ -524,6 +559,14 @@ int orc_make_back_neighs(int natoms, int nnode, const int* neighs, int* bk, int*
         }
     }
     return 0;
+}
+
+void orc_set_restraints(int natoms, const double* constr, const double* constrK, const double* bbox9) {
+    g_restraints.natoms = natoms;
+    if (natoms <= 0) return;
+    g_restraints.constr.assign(constr, constr + natoms * 4);
+    g_restraints.constrK.assign(constrK, constrK + natoms * 4);
+    for (int k = 0; k < 9; k++) g_restraints.bbox[k] = bbox9 ? bbox9[k] : 0.0;
 }
 
 double orc_eval(int natoms, int nnode, const double* apos, const double* REQs, const int* neighs, const double* apars,

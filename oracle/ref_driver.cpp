@@ -18,6 +18,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 namespace {
 
@@ -247,6 +248,67 @@ void ref_sample_pair(int n, const double* rs, const double* REQij, double* Es, d
         Es[i] = getLJQH(Vec3d{rs[i], 0, 0}, f, q, 1.0);
         Fs[i] = -f.x;
     }
+}
+
+// MMFFsp3_loc::initPi (MMFFsp3_loc.h:882-945) on a non-periodic molecule
+// (every neighbour in cell 0 with a zero shift), optionally followed by
+// flipPis(Vec3dOne) (MMFFsp3_loc.h:1167-1172) as MolWorld_sp3 does after
+// building (MolWorld_sp3.h:1400).  pipos_out: nnode x 3.
+void ref_init_pi(int natoms, int nnode, const double* apos, const int* neighs, const double* Ksp, const double* Kpp,
+                 int flip, double* pipos_out) {
+    MMFFsp3_loc ffl;
+    ffl.realloc(nnode, natoms - nnode);
+    ffl.bPBC = false;
+    for (int i = 0; i < ffl.nvecs; i++) ffl.apos[i] = Vec3dZero;
+    for (int i = 0; i < natoms; i++) {
+        ffl.apos[i] = Vec3d{apos[i * 3], apos[i * 3 + 1], apos[i * 3 + 2]};
+        ffl.neighs[i] = Quat4i{neighs[i * 4], neighs[i * 4 + 1], neighs[i * 4 + 2], neighs[i * 4 + 3]};
+        ffl.neighCell[i] = Quat4i{0, 0, 0, 0};
+    }
+    for (int i = 0; i < nnode; i++) {
+        ffl.Ksp[i] = Quat4d{Ksp[i * 4], Ksp[i * 4 + 1], Ksp[i * 4 + 2], Ksp[i * 4 + 3]};
+        ffl.Kpp[i] = Quat4d{Kpp[i * 4], Kpp[i * 4 + 1], Kpp[i * 4 + 2], Kpp[i * 4 + 3]};
+    }
+    Vec3d shift0 = Vec3dZero;
+    ffl.initPi(&shift0);
+    if (flip) ffl.flipPis(Vec3dOne);
+    for (int i = 0; i < nnode; i++) {
+        pipos_out[i * 3] = ffl.pipos[i].x;
+        pipos_out[i * 3 + 1] = ffl.pipos[i].y;
+        pipos_out[i * 3 + 2] = ffl.pipos[i].z;
+    }
+}
+
+// GridFF node potentials of a substrate at npts points with the reference's
+// own GridFF::evalGridFFPoint_Mors (GridFF.h:582-611) over the images of
+// makePBCshifts_({nmx,nmy,0}) and evalGridFFPoint_Coul (:614-635, bare) over
+// those of ({ncx,ncy,0}) (NBFF.h:42-57): out[i] = (Pauli, London, Coulomb).
+void ref_gridff_points(int natoms, const double* apos, const double* REQs, const double* lvec9, double alpha,
+                       int nmx, int nmy, int ncx, int ncy, int npts, const double* pts, double* out) {
+    GridFF gff;
+    gff.alphaMorse = alpha;
+    Mat3d lvec{lvec9[0], lvec9[1], lvec9[2], lvec9[3], lvec9[4], lvec9[5], lvec9[6], lvec9[7], lvec9[8]};
+    Vec3d* sm = nullptr;
+    Vec3d* sc = nullptr;
+    const int nm = makePBCshifts_(Vec3i{nmx, nmy, 0}, lvec, sm);
+    const int nc = makePBCshifts_(Vec3i{ncx, ncy, 0}, lvec, sc);
+    std::vector<Vec3d> ap(natoms);
+    std::vector<Quat4d> rq(natoms);
+    for (int i = 0; i < natoms; i++) {
+        ap[i] = Vec3d{apos[i * 3], apos[i * 3 + 1], apos[i * 3 + 2]};
+        rq[i] = Quat4d{REQs[i * 4], REQs[i * 4 + 1], REQs[i * 4 + 2], REQs[i * 4 + 3]};
+    }
+    for (int i = 0; i < npts; i++) {
+        const Vec3d p{pts[i * 3], pts[i * 3 + 1], pts[i * 3 + 2]};
+        Quat4d qp, ql;
+        gff.evalGridFFPoint_Mors(nm, sm, natoms, ap.data(), rq.data(), p, qp, ql);
+        const Quat4d qe = gff.evalGridFFPoint_Coul(nc, sc, natoms, ap.data(), rq.data(), p);
+        out[i * 3] = qp.w;
+        out[i * 3 + 1] = ql.w;
+        out[i * 3 + 2] = qe.w;
+    }
+    delete[] sm;
+    delete[] sc;
 }
 
 }  // extern "C"

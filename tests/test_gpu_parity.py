@@ -558,3 +558,44 @@ def test_pair_systems_forces(nat, orc):
     assert np.abs(f_near - f_far).max() > 1e-3
     nat.relax(0, 1e-12, 0, True, True)
     check_forces(nat, orc, nat.Batch(), grid=g)
+
+
+# ------------------------------------------------------------------ GridFF generation, pi-vectors
+def test_gridff_device_nodes_match_host(nat, nacl_grid):
+    """The FP64 device kernel of the GridFF node potentials reproduces the host
+    definition (pinned against the reference's evalGridFFPoint_Mors/_Coul on
+    the CPU), and the grid that init builds with it (device nodes, host fit)
+    is the host-built grid."""
+    import ctypes
+    P("case: GridFF generation on the device")
+    L = nat.lib()
+    L.gridff_nacl_nodes.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    L.gridff_nacl_nodes.restype = ctypes.c_int
+    nx, ny, nz, dz = 28, 28, 60, 0.1
+    Vh = np.zeros((nx, ny, nz, 3))
+    Vd = np.zeros((nx, ny, nz, 3))
+    assert L.gridff_nacl_nodes(nx, ny, nz, dz, 0, Vh.ctypes.data) == 0
+    assert L.gridff_nacl_nodes(nx, ny, nz, dz, 1, Vd.ctypes.data) == 0
+    assert np.abs(Vd - Vh).max() < 1e-12 * (1 + np.abs(Vh).max()), np.abs(Vd - Vh).max()
+    nat.init_workload(1, 2, 10001, 16, 32, 0.1, "synth:NaCl", 0.1)
+    g = nat.grid_info()
+    assert g["n"] == nacl_grid["n"]
+    assert np.allclose(g["pos0"], nacl_grid["pos0"]) and np.allclose(g["step"], nacl_grid["step"])
+    c0, c1 = np.asarray(g["coefs"]), nacl_grid["coefs"]
+    assert np.abs(c0 - c1).max() < 1e-9 * (1 + np.abs(c1).max())
+
+
+def test_init_systems_derives_missing_pi_vectors(nat):
+    """Systems given without pi-vectors (zero rows) get them from the geometry
+    (MMFFsp3_loc::initPi + flipPis) before upload."""
+    P("case: init_systems pi-vectors")
+    mols = [nat.gen_molecule(n, s, 0.1, s + 3) for n, s in [(30, 41), (90, 42)]]
+    for m in mols:
+        m["apos"] = np.array(m["apos"])
+        m["apos"][m["natoms"]:] = 0.0
+    nat.init_systems(mols)
+    B = nat.Batch()
+    for s, m in enumerate(mols):
+        N = m["natoms"]
+        exp = nat.init_pi_vectors(m["apos"][:N], m["neighs"], m["nnode"], m["Ksp"], m["Kpp"])
+        assert np.abs(B.positions(s)[N:] - exp).max() < 1e-6, s

@@ -1,0 +1,121 @@
+"""GridFF generation (node potentials + B-spline fit), CPU checks.
+
+* node potentials of the host definition against the golden fixture written
+  from the reference's own GridFF::evalGridFFPoint_Mors / evalGridFFPoint_Coul
+  (and against that build live where it is present);
+* the fit is an exact interpolation: the FP64 oracle's B-spline sample
+  (fe3d_pbc_comb3 restated) at every interior node returns the node value;
+* the lower bound on r^2 at the nodes that coincide with ions (where the
+  reference's r^2 + 1e-32 would give ~1e17 eV) does not reach the heights where
+  molecules sit: changing those node values moves the interpolated potential
+  at z >= 2 A by less than 1e-10 of the change.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+import oracle
+from firecore_amd import native
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "gridff_points.npz")
+
+
+def _lib():
+    L = native.lib()
+    L.gridff_point_values.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_double, ctypes.c_int,
+                                                                                ctypes.c_void_p, ctypes.c_void_p]
+    L.gridff_point_values.restype = ctypes.c_int
+    L.gridff_fit_values.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 2
+    L.gridff_fit_values.restype = ctypes.c_int
+    L.gridff_nacl_nodes.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    L.gridff_nacl_nodes.restype = ctypes.c_int
+    return L
+
+
+def point_values(apos, REQs, lvec, alpha, pts):
+    L = _lib()
+    apos, REQs = np.ascontiguousarray(apos, np.float64), np.ascontiguousarray(REQs, np.float64)
+    lvec, pts = np.ascontiguousarray(lvec, np.float64), np.ascontiguousarray(pts, np.float64)
+    out = np.zeros((len(pts), 3))
+    assert L.gridff_point_values(len(apos), apos.ctypes.data, REQs.ctypes.data, lvec.ctypes.data, alpha, len(pts),
+                                 pts.ctypes.data, out.ctypes.data) == 0
+    return out
+
+
+def fit(V):
+    L = _lib()
+    V = np.ascontiguousarray(V, np.float64)
+    c = np.zeros_like(V)
+    assert L.gridff_fit_values(V.shape[0], V.shape[1], V.shape[2], V.ctypes.data, c.ctypes.data) == 0
+    return c
+
+
+def test_node_potentials_match_reference_golden():
+    z = np.load(GOLD)
+    got = point_values(z["apos"], z["REQs"], z["lvec"], float(z["alpha"]), z["pts"])
+    ref = z["PLQ"]
+    for ch in range(3):
+        scale = 1.0 + np.abs(ref[:, ch]).max()
+        assert np.abs(got[:, ch] - ref[:, ch]).max() < 1e-11 * scale, ch
+
+
+def test_node_potentials_match_reference_build():
+    z = np.load(GOLD)
+    rng = np.random.default_rng(9)
+    pts = np.stack([rng.uniform(-3, 3, 40), rng.uniform(-3, 3, 40), rng.uniform(0.1, 6, 40)], 1)
+    ref = oracle.ref_gridff_points(z["apos"], z["REQs"], z["lvec"], 1.5, (3, 3), (25, 25), pts)
+    if ref is None:
+        import pytest
+        pytest.skip("reference build oracle/_ref is not present on this machine")
+    got = point_values(z["apos"], z["REQs"], z["lvec"], 1.5, pts)
+    assert np.abs(got - ref).max() < 1e-11 * (1 + np.abs(ref).max())
+
+
+def test_fit_interpolates_nodes(orc):
+    nx, ny, nz = 12, 10, 18
+    rng = np.random.default_rng(4)
+    X, Y, Z = np.meshgrid(np.arange(nx) / nx, np.arange(ny) / ny, np.arange(nz), indexing="ij")
+    V = np.stack([np.cos(2 * np.pi * X) * np.exp(-0.3 * Z), np.sin(2 * np.pi * Y) + 0.01 * Z ** 2,
+                  rng.normal(size=X.shape)], -1)
+    C = fit(V)
+    step = np.array([0.3, 0.25, 0.2])
+    g = dict(n=(nx, ny, nz), pos0=np.array([-1.0, 0.5, 0.0]), step=step, alpha=1.5, coefs=C)
+    worst = 0.0
+    for ix in range(nx):
+        for iy in range(0, ny, 3):
+            for iz in range(3, nz - 4):
+                p = g["pos0"] + step * np.array([ix, iy, iz]) + 1e-12
+                for ch in range(3):
+                    plq = np.zeros(4)
+                    plq[ch] = 1.0
+                    e, _ = orc.grid_sample(g, p, plq)
+                    worst = max(worst, abs(e - V[ix, iy, iz, ch]) / (1 + abs(V[ix, iy, iz, ch])))
+    assert worst < 1e-10, worst
+
+
+def test_ion_node_bound_does_not_reach_molecule_heights(orc):
+    L = _lib()
+    nx, ny, nz, dz = 28, 28, 40, 0.1
+    V = np.zeros((nx, ny, nz, 3))
+    assert L.gridff_nacl_nodes(nx, ny, nz, dz, 0, V.ctypes.data) == 0
+    assert np.isfinite(V).all()
+    # nodes of the top plane that coincide with ions (x, y on multiples of a/2)
+    hot = np.zeros((nx, ny), bool)
+    for ix in (0, nx // 2):
+        for iy in (0, ny // 2):
+            hot[ix, iy] = True
+    assert np.abs(V[hot, 0, 2]).min() > 1e3  # the bounded 1/r at an ion: k q / 0.01 A
+    V2 = V.copy()
+    V2[hot, 0, 2] += 1e3
+    C1, C2 = fit(V), fit(V2)
+    a = 5.64
+    g1 = dict(n=(nx, ny, nz), pos0=np.array([-a / 2, -a / 2, 0.0]), step=np.array([a / nx, a / ny, dz]), alpha=1.5,
+              coefs=C1)
+    g2 = dict(g1, coefs=C2)
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        p = np.array([rng.uniform(-a / 2, a / 2), rng.uniform(-a / 2, a / 2), rng.uniform(2.0, 3.5)])
+        e1, _ = orc.grid_sample(g1, p, np.array([0, 0, 1.0, 0]))
+        e2, _ = orc.grid_sample(g2, p, np.array([0, 0, 1.0, 0]))
+        assert abs(e1 - e2) < 1e-10 * 1e3, (p, e1 - e2)
