@@ -62,7 +62,8 @@ bool setup_surface(const char* surf_name, double gridStep, fc::GridFFHost& g) {
     std::string base;
     if (sn.rfind("synth:NaCl", 0) == 0) {
         fc::make_nacl_slab(5.64, 2, 0.9, S);
-        base = "NaCl_synth";
+        S.Rdamp = fc::NACL_SYNTH_RDAMP;
+        base = "NaCl_synth_Rdamp1";
     } else {
         std::string fx = sn;
         if (fx.size() < 4 || fx.substr(fx.size() - 4) != ".xyz") fx += ".xyz";
@@ -1153,9 +1154,10 @@ int gridff_point_values(int natoms, const double* apos, const double* REQs, cons
 // Node potentials of the synthetic NaCl(001) GridFF grid (origin at the top
 // atomic plane, as make_nacl_gridff) computed by the device kernel (device=1)
 // or the host (device=0), (nx,ny,nz,3) z fastest.
-int gridff_nacl_nodes(int nx, int ny, int nz, double dz, int device, double* out) {
+int gridff_nacl_nodes(int nx, int ny, int nz, double dz, double Rdamp, int device, double* out) {
     fc::Surface S;
     fc::make_nacl_slab(5.64, 2, 0.9, S);
+    S.Rdamp = Rdamp;
     fc::GridFFHost g;
     fc::gridff_geometry(S, nx, ny, nz, dz, 0.0, 1.5, g);
     if (device) {
@@ -1182,9 +1184,11 @@ int gridff_fit_values(int nx, int ny, int nz, const double* V, double* coefs) {
     return 0;
 }
 
-int make_nacl_gridff(int nx, int ny, int nz, double dz, int nthreads, double* coefs_out, double* geom_out7) {
+int make_nacl_gridff(int nx, int ny, int nz, double dz, double Rdamp, int nthreads, double* coefs_out,
+                     double* geom_out7) {
     fc::Surface S;
     fc::make_nacl_slab(5.64, 2, 0.9, S);
+    S.Rdamp = Rdamp;
     fc::GridFFHost g;
     fc::gridff_geometry(S, nx, ny, nz, dz, 0.0, 1.5, g);
     std::vector<double> V;

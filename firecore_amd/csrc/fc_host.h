@@ -109,15 +109,25 @@ bool npy_save_f64(const std::string& fname, const std::vector<int>& shape, const
 struct Surface {
     std::vector<double> apos, REQs;
     double lvec[9] = {0};
+    // Coulomb channel of the node sums: 0 = bare k*q/r (the reference's
+    // evalGridFFPoint_Coul, r^2 bounded below by GRIDFF_R2MIN); > 0 = damped
+    // k*q/sqrt(r^2 + Rdamp^2), the form of the reference's GPU Coulomb
+    // (getLJQH with R2damp, relax_multi.cl:151-164)
+    double Rdamp = 0.0;
     int natoms() const { return (int)(apos.size() / 3); }
 };
 void make_nacl_slab(double a, int nlayers, double q, Surface& s);
 bool load_surface_xyz(const std::string& fname, Surface& s);
 bool save_surface_xyz(const std::string& fname, const Surface& s);
-// GridFF generation (gridff_host.cpp): node potentials (Pauli, London, bare
+// GridFF generation (gridff_host.cpp): node potentials (Pauli, London,
 // Coulomb) as the reference's GridFF::evalBsplineRef, then exact cubic
 // B-spline interpolation (periodic in x,y, open in z).
-constexpr double GRIDFF_R2MIN = 1e-4;  // (0.01 A)^2: lower bound of r^2 in the Coulomb node sums
+constexpr double GRIDFF_R2MIN = 1e-4;  // (0.01 A)^2: lower bound of r^2 in the bare Coulomb node sums
+// Coulomb damping of the synthetic NaCl(001) substrate of the benchmark
+// ("synth:NaCl"): its molecules carry random charges (hydrogens up to
+// ~0.5 e), which a bare ionic potential pulls onto the top-layer ions and,
+// below the grid's first nodes, through the slab
+constexpr double NACL_SYNTH_RDAMP = 1.0;
 void gridff_images(const Surface& s, int& nmx, int& nmy, int& ncx, int& ncy);
 void gridff_point(const Surface& s, const double* p, double alpha, int nmx, int nmy, int ncx, int ncy, double* out);
 void gridff_geometry(const Surface& s, int nx, int ny, int nz, double dz, double z0, double alpha, GridFFHost& g);

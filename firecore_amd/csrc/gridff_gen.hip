@@ -18,6 +18,7 @@ struct GridGenDev {
     double ax, ay, bx, by;  // lateral lattice vectors
     double alpha;
     int nmx, nmy, ncx, ncy;  // Morse and Coulomb image ranges
+    double rdamp2;           // Coulomb damping Rdamp^2 (0 = bare, r^2 bounded below)
 };
 
 __global__ void __launch_bounds__(256) k_gridff_nodes(GridGenDev G, const double* __restrict__ apos,
@@ -50,7 +51,8 @@ __global__ void __launch_bounds__(256) k_gridff_nodes(GridGenDev G, const double
             for (int jx = -G.ncx; jx <= G.ncx; jx++) {
                 const double dx = px - Ax + jx * G.ax + jy * G.bx;
                 const double dy = py - Ay + jx * G.ay + jy * G.by;
-                const double r2 = fmax(dx * dx + dy * dy + dz * dz, fc::GRIDFF_R2MIN);
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                const double r2 = (G.rdamp2 > 0.0) ? d2 + G.rdamp2 : fmax(d2, fc::GRIDFF_R2MIN);
                 Q += kq / sqrt(r2);
             }
     }
@@ -87,6 +89,7 @@ extern "C" hipError_t fc_gridff_nodes_device(const fc::Surface* s, const fc::Gri
     G.nmy = nmy;
     G.ncx = ncx;
     G.ncy = ncy;
+    G.rdamp2 = s->Rdamp > 0.0 ? s->Rdamp * s->Rdamp : 0.0;
     const size_t np = (size_t)G.nx * G.ny * G.nz;
     if (np == 0 || G.na == 0) return hipErrorInvalidValue;
     double *d_apos = nullptr, *d_req = nullptr, *d_V = nullptr;

@@ -258,7 +258,8 @@ void gridff_point(const Surface& s, const double* p, double alpha, int nmx, int 
             for (int jx = -ncx; jx <= ncx; jx++) {
                 const double dx = p[0] - A[0] + jx * ax + jy * bx, dy = p[1] - A[1] + jx * ay + jy * by,
                              dz = p[2] - A[2];
-                const double r2 = std::max(dx * dx + dy * dy + dz * dz, GRIDFF_R2MIN);
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                const double r2 = (s.Rdamp > 0.0) ? d2 + s.Rdamp * s.Rdamp : std::max(d2, GRIDFF_R2MIN);
                 Q += COULOMB_K * q / std::sqrt(r2);
             }
     }

@@ -58,7 +58,7 @@ def lib():
     L.make_back_neighs.restype = c_int
     L.make_exclusions.argtypes = [c_int, c_int, c_int_p, c_int, ctypes.POINTER(ctypes.c_uint), c_int_p, c_int_p]
     L.make_exclusions.restype = c_int
-    L.make_nacl_gridff.argtypes = [c_int, c_int, c_int, c_double, c_int, c_double_p, c_double_p]
+    L.make_nacl_gridff.argtypes = [c_int, c_int, c_int, c_double, c_double, c_int, c_double_p, c_double_p]
     L.make_nacl_gridff.restype = c_int
     L.getfBuff.argtypes = [c_char_p]
     L.getfBuff.restype = c_float_p
@@ -167,11 +167,12 @@ def make_exclusions(natoms, nnode, neighs, subtract13=True):
     return bits.astype(bool), nc.value
 
 
-def make_nacl_grid(nx=56, ny=56, nz=200, dz=0.1, nthreads=8):
-    """Synthetic NaCl(001) GridFF coefficients (host direct sums + fit)."""
+def make_nacl_grid(nx=56, ny=56, nz=200, dz=0.1, nthreads=8, Rdamp=1.0):
+    """Synthetic NaCl(001) GridFF coefficients (host direct sums + fit); Rdamp =
+    Coulomb damping of the node sums (1 A: the benchmark substrate; 0: bare)."""
     coefs = np.zeros((nx, ny, nz, 3))
     geom = np.zeros(7)
-    lib().make_nacl_gridff(nx, ny, nz, dz, nthreads, _p(coefs), _p(geom))
+    lib().make_nacl_gridff(nx, ny, nz, dz, Rdamp, nthreads, _p(coefs), _p(geom))
     return dict(n=(nx, ny, nz), pos0=geom[0:3].copy(), step=geom[3:6].copy(), alpha=float(geom[6]), coefs=coefs)
 
 

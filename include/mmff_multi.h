@@ -286,19 +286,25 @@ int gen_molecule_arrays(int natoms, long long seed, double sigma, long long psee
 int make_back_neighs(int natoms, int nnode, const int* neighs, int* bkneighs);
 int make_exclusions(int natoms, int nnode, const int* neighs, int subtract13, unsigned* mask_out, int* W_out,
                     int* ncorr_out);
-int make_nacl_gridff(int nx, int ny, int nz, double dz, int nthreads, double* coefs_out, double* geom_out7);
+/* Synthetic NaCl(001) slab GridFF (host sums + fit); Rdamp = Coulomb damping
+ * of the node sums (0 = bare; the benchmark substrate "synth:NaCl" uses 1 A). */
+int make_nacl_gridff(int nx, int ny, int nz, double dz, double Rdamp, int nthreads, double* coefs_out,
+                     double* geom_out7);
 /* GridFF generation (the reference's GridFF::evalBsplineRef node potentials,
  * GridFF.h:848-890: Pauli and London over the Morse images within 20 A, bare
  * Coulomb over images -25..25, r^2 >= 1e-4 A^2; then exact cubic B-spline
  * interpolation, periodic in x,y, open in z).  init()/init_workload compute the
- * node potentials on the device and fit on the host.
+ * node potentials on the device and fit on the host.  Substrates loaded from
+ * .xyz files get the bare Coulomb channel; the synthetic benchmark substrate
+ * ("synth:NaCl") the damped one, k*q/sqrt(r^2 + 1 A^2).
  *   gridff_point_values: (P, L, Q) of a substrate at npts points, host FP64
- *   gridff_nacl_nodes:   node potentials of the synthetic NaCl(001) grid on the
- *                        device (device=1) or the host (0)
+ *   gridff_nacl_nodes:   node potentials of the synthetic NaCl(001) grid with
+ *                        Coulomb damping Rdamp (0 = bare) on the device
+ *                        (device=1) or the host (0)
  *   gridff_fit_values:   interpolation coefficients of node values V */
 int gridff_point_values(int natoms, const double* apos, const double* REQs, const double* lvec9, double alpha,
                         int npts, const double* pts, double* out);
-int gridff_nacl_nodes(int nx, int ny, int nz, double dz, int device, double* out);
+int gridff_nacl_nodes(int nx, int ny, int nz, double dz, double Rdamp, int device, double* out);
 int gridff_fit_values(int nx, int ny, int nz, const double* V, double* coefs);
 
 #ifdef __cplusplus

@@ -569,14 +569,16 @@ def test_gridff_device_nodes_match_host(nat, nacl_grid):
     import ctypes
     P("case: GridFF generation on the device")
     L = nat.lib()
-    L.gridff_nacl_nodes.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    L.gridff_nacl_nodes.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                                          ctypes.c_void_p]
     L.gridff_nacl_nodes.restype = ctypes.c_int
     nx, ny, nz, dz = 28, 28, 60, 0.1
-    Vh = np.zeros((nx, ny, nz, 3))
-    Vd = np.zeros((nx, ny, nz, 3))
-    assert L.gridff_nacl_nodes(nx, ny, nz, dz, 0, Vh.ctypes.data) == 0
-    assert L.gridff_nacl_nodes(nx, ny, nz, dz, 1, Vd.ctypes.data) == 0
-    assert np.abs(Vd - Vh).max() < 1e-12 * (1 + np.abs(Vh).max()), np.abs(Vd - Vh).max()
+    for rdamp in (0.0, 1.0):  # bare (reference) and damped (benchmark substrate) Coulomb channel
+        Vh = np.zeros((nx, ny, nz, 3))
+        Vd = np.zeros((nx, ny, nz, 3))
+        assert L.gridff_nacl_nodes(nx, ny, nz, dz, rdamp, 0, Vh.ctypes.data) == 0
+        assert L.gridff_nacl_nodes(nx, ny, nz, dz, rdamp, 1, Vd.ctypes.data) == 0
+        assert np.abs(Vd - Vh).max() < 1e-12 * (1 + np.abs(Vh).max()), (rdamp, np.abs(Vd - Vh).max())
     nat.init_workload(1, 2, 10001, 16, 32, 0.1, "synth:NaCl", 0.1)
     g = nat.grid_info()
     assert g["n"] == nacl_grid["n"]

@@ -28,7 +28,8 @@ def _lib():
     L.gridff_point_values.restype = ctypes.c_int
     L.gridff_fit_values.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 2
     L.gridff_fit_values.restype = ctypes.c_int
-    L.gridff_nacl_nodes.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+    L.gridff_nacl_nodes.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                                          ctypes.c_void_p]
     L.gridff_nacl_nodes.restype = ctypes.c_int
     return L
 
@@ -98,7 +99,7 @@ def test_ion_node_bound_does_not_reach_molecule_heights(orc):
     L = _lib()
     nx, ny, nz, dz = 28, 28, 40, 0.1
     V = np.zeros((nx, ny, nz, 3))
-    assert L.gridff_nacl_nodes(nx, ny, nz, dz, 0, V.ctypes.data) == 0
+    assert L.gridff_nacl_nodes(nx, ny, nz, dz, 0.0, 0, V.ctypes.data) == 0
     assert np.isfinite(V).all()
     # nodes of the top plane that coincide with ions (x, y on multiples of a/2)
     hot = np.zeros((nx, ny), bool)
@@ -119,3 +120,34 @@ def test_ion_node_bound_does_not_reach_molecule_heights(orc):
         e1, _ = orc.grid_sample(g1, p, np.array([0, 0, 1.0, 0]))
         e2, _ = orc.grid_sample(g2, p, np.array([0, 0, 1.0, 0]))
         assert abs(e1 - e2) < 1e-10 * 1e3, (p, e1 - e2)
+
+
+def test_damped_coulomb_channel_matches_direct_sum():
+    """The benchmark substrate's Coulomb channel, k*q/sqrt(r^2 + Rdamp^2) over
+    images -25..25 of the 8-ion NaCl(001) cell (two layers, a = 5.64 A, +-0.9 e),
+    against a numpy direct sum at a few nodes; Rdamp = 0 gives the bare sum
+    (bounded r^2 >= 1e-4 A^2) and the other channels do not depend on Rdamp."""
+    L = _lib()
+    nx, ny, nz, dz = 8, 8, 12, 0.25
+    V0 = np.zeros((nx, ny, nz, 3))
+    V1 = np.zeros((nx, ny, nz, 3))
+    assert L.gridff_nacl_nodes(nx, ny, nz, dz, 0.0, 0, V0.ctypes.data) == 0
+    assert L.gridff_nacl_nodes(nx, ny, nz, dz, 1.0, 0, V1.ctypes.data) == 0
+    assert np.array_equal(V0[..., :2], V1[..., :2])
+    a, q, k = 5.64, 0.9, 14.3996448915
+    ions = []
+    for layer in range(2):
+        for kk, (x, y) in enumerate([(0, 0), (0.5, 0.5), (0.5, 0), (0, 0.5)]):
+            na = (kk < 2) ^ (layer & 1)
+            ions.append((x * a, y * a, -0.5 * a * layer, q if na else -q))
+    ions = np.array(ions)
+    jx, jy = np.meshgrid(np.arange(-25, 26), np.arange(-25, 26), indexing="ij")
+    for ix, iy, iz in [(1, 2, 3), (0, 0, 5), (5, 3, 11), (7, 7, 8)]:
+        p = np.array([-a / 2 + ix * a / nx, -a / 2 + iy * a / ny, iz * dz])
+        for rd, V in ((0.0, V0), (1.0, V1)):
+            tot = 0.0
+            for X, Y, Z, Q in ions:
+                d2 = (p[0] - X + jx * a) ** 2 + (p[1] - Y + jy * a) ** 2 + (p[2] - Z) ** 2
+                r2 = d2 + rd * rd if rd > 0 else np.maximum(d2, 1e-4)
+                tot += (k * Q / np.sqrt(r2)).sum()
+            assert abs(V[ix, iy, iz, 2] - tot) < 1e-9 * (1 + abs(tot)), (ix, iy, iz, rd, V[ix, iy, iz, 2], tot)
