@@ -788,6 +788,19 @@ int relax(int nstepMax, double Fconv, int path, bool restore_start, bool bDownlo
     return n;
 }
 
+// relax() path 0 with no device allocation: the persistent kernel's scratch
+// (queue counters and order, network activation slots, GridFF window cache) in
+// the library's static device array of FC_NOALLOC_SCRATCH_BYTES.  Rejects
+// (returns -1 with a message) batches that need more scratch or a path that
+// allocates.  Calls that overlap on one device share its scratch and race.
+int relax_noalloc(int nstepMax, double Fconv, bool restore_start, bool bDownload) {
+    World& W = world();
+    if (!W.dev_ready || W.nSys == 0) return -1;
+    int n = W.relax_noalloc(nstepMax, Fconv, restore_start);
+    if (n >= 0 && bDownload && !W.download_all(true, true)) return -1;
+    return n;
+}
+
 double get_last_ms() { return world().last_ms; }
 
 int set_cost_model(int feedback, double c0, double c1) {

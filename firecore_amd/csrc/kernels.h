@@ -4,6 +4,10 @@
 
 #include "ff_device.h"
 
+// Size of the device scratch array of the allocation-free relaxation (queue
+// counters, queue order, network activation slots, GridFF window cache).
+#define FC_NOALLOC_SCRATCH_BYTES (64u << 20)
+
 namespace fcd {
 
 // One relaxation system in the CSR-packed batch.

@@ -659,6 +659,33 @@ extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups,
     return hipLaunchKernel(k, dim3(ngroups), dim3(block), args, lds_bytes, stream);
 }
 
+// Scratch of the allocation-free relaxation (World::relax_noalloc): the queue
+// counters, the queue order, the activation slots of the in-kernel network and
+// the GridFF window cache of one relaxation, in one file-scope device array of
+// FC_NOALLOC_SCRATCH_BYTES.  Each device holds its own copy at its own address
+// until the process ends; host code uses the address returned here, never the
+// array's name.
+static __device__ __align__(256) unsigned char scratch[FC_NOALLOC_SCRATCH_BYTES];
+
+// Device address of this device's scratch array (looked up once per device,
+// with that device current).
+extern "C" hipError_t fc_noalloc_scratch(void** ptr) {
+    constexpr int kMaxDev = 64;
+    static void* dev_ptr[kMaxDev] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDev) return hipErrorInvalidDevice;
+    if (!dev_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        dev_ptr[dev] = p;
+    }
+    *ptr = dev_ptr[dev];
+    return hipSuccess;
+}
+
 extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block, int learned, int h2lds) {
     fcd::LdsLayout L{nmax, mmax, wmax, block / 64, learned, h2lds};
     return L.bytes();

@@ -19,6 +19,8 @@ extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups,
                                                  const int* qinfo, int* counters, int cls, int nmax, int mmax,
                                                  int wmax, int steal, unsigned short* hslot, int h2lds);
 extern "C" int fc_hslot_elems(int nmax);
+extern "C" hipError_t fc_noalloc_scratch(void** ptr);
+extern "C" hipError_t fc_launch_zero_int(hipStream_t st, int* a, int n);
 extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block, int learned, int h2lds);
 extern "C" hipError_t fc_launch_split_step(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P,
                                            const int* node_sys, int ntot_nodes, int maxN, int do_move, int step);
@@ -291,6 +293,14 @@ bool World::ensure_device() {
         if (!hip_ok(hipStreamCreate(&st), "hipStreamCreate")) return false;
         class_streams.push_back(st);
     }
+    // allocation-free relaxation: this device's scratch array and the fork/join
+    // events of its class launches, once at set-up
+    if (!hip_ok(fc_noalloc_scratch(&noalloc_ptr), "scratch address")) return false;
+    for (int i = 0; i < 10; i++) {
+        hipEvent_t e = nullptr;
+        if (!hip_ok(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate")) return false;
+        noalloc_events.push_back(e);
+    }
     dev_ready = true;
     return true;
 }
@@ -495,7 +505,7 @@ fcd::RunParams World::run_params(double Fconv, int nstep_max) {
 // estimated time; workgroups that run out of work steal from smaller classes.
 // The estimate uses the previous relaxation's step counts of the same batch
 // when available (the autotuning loop), else a size-based guess.
-void World::plan_classes() {
+void World::plan_classes(bool upload) {
     classes.clear();
     const int blocks[4] = {64, 128, 192, 256};
     std::vector<SizeClass> cl(4);
@@ -600,7 +610,8 @@ void World::plan_classes() {
         qinfo[4 + c] = qcnt[c];
     }
     if (allorder.empty()) allorder.push_back(0);
-    d_order.upload(allorder, stream);
+    h_order = allorder;
+    if (upload) d_order.upload(allorder, stream);
 }
 
 int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
@@ -668,6 +679,87 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
     d_nsteps.download(nsteps, stream);
     d_status.download(status, stream);
     hip_ok(hipStreamSynchronize(stream), "relax results");
+    int mx = 0;
+    for (int s = 0; s < nSys; s++) mx = std::max(mx, nsteps[s]);
+    return mx;
+}
+
+// relax_persistent without allocation: the queue counters, the queue order, the
+// in-kernel network's activation slots and the GridFF window cache of this
+// relaxation live in the library's static scratch array (noalloc_ptr, looked up
+// at set-up); the class launches fork and join through events made at set-up.
+// Calls no hipMalloc/hipFree.  Paths that need more (systems above the largest
+// workgroup class, asymmetric exclusions, the chunked learned form, per-system
+// timing) and batches whose scratch exceeds FC_NOALLOC_SCRATCH_BYTES are
+// rejected with a message.  Every call on a device shares that device's array:
+// calls that overlap on one device race.
+int World::relax_noalloc(int nstep_max, double Fconv, bool restore_start) {
+    auto fail = [](const char* why) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] relax_noalloc: %s\n", why);
+        return -1;
+    };
+    if (!dev_ready || nSys == 0 || !noalloc_ptr) return fail("not initialised");
+    if (pc_mode == 2 && !learned_inkernel) return fail("the chunked learned form allocates; use relax()");
+    if (pc_mode == 2 && !d_mlpW.p) return fail("learned gains need proposal weights");
+    if (!ring_ok) return fail("asymmetric exclusions need the split kernels; use relax()");
+    if (record_times) return fail("per-system timing allocates; switch it off (set_record_times(0))");
+    if (autotune || h_order.empty()) plan_classes(false);
+    if (n_unclassed > 0) return fail("systems above 256 atoms need the split kernels; use relax()");
+    // scratch layout: counters | order | activation slots | window cache (256 B aligned)
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t off_order = 256;
+    const size_t off_hslot = off_order + up(h_order.size() * sizeof(int));
+    std::vector<size_t> slot_off(classes.size(), 0);
+    size_t nslot = 0;
+    if (pc_mode == 2)
+        for (size_t k = 0; k < classes.size(); k++) {
+            slot_off[k] = nslot;
+            nslot += (size_t)classes[k].ngroups * fc_hslot_elems(std::max(classes[k].nmax, 1));
+        }
+    const size_t off_cache = off_hslot + up(nslot * sizeof(unsigned short));
+    const size_t need = off_cache + (size_t)ntotA * 16 * sizeof(float4);
+    if (need > FC_NOALLOC_SCRATCH_BYTES) {
+        std::fprintf(stderr,
+                     "[libMMFFmulti_lib] relax_noalloc: the batch needs %zu bytes of scratch (%d atoms, %d systems), "
+                     "more than the %u available; use relax()\n",
+                     need, ntotA, nSys, (unsigned)FC_NOALLOC_SCRATCH_BYTES);
+        return -1;
+    }
+    char* base = static_cast<char*>(noalloc_ptr);
+    int* counters = reinterpret_cast<int*>(base);
+    int* order = reinterpret_cast<int*>(base + off_order);
+    unsigned short* hslot = reinterpret_cast<unsigned short*>(base + off_hslot);
+    if (restore_start &&
+        !hip_ok(hipMemcpyAsync(d_atoms.p, d_atoms0.p, sizeof(float4) * ntotV, hipMemcpyDeviceToDevice, stream),
+                "restore"))
+        return -1;
+    if (!hip_ok(hipMemcpyAsync(order, h_order.data(), h_order.size() * sizeof(int), hipMemcpyHostToDevice, stream),
+                "queue order") ||
+        !hip_ok(fc_launch_zero_int(stream, counters, 8), "queue reset"))
+        return -1;
+    fcd::BatchDev B = batch_dev();
+    B.systime = nullptr;
+    B.gridcache = reinterpret_cast<float4*>(base + off_cache);
+    B.gridcache_stride = ntotA;
+    fcd::RunParams P = run_params(Fconv, nstep_max);
+    hipEvent_t fork = noalloc_events[0];
+    hipEventRecord(fork, stream);
+    for (size_t k = 0; k < classes.size() && k < noalloc_events.size() - 1; k++) {
+        const SizeClass& C = classes[k];
+        hipStream_t st = class_streams[k % class_streams.size()];
+        hipStreamWaitEvent(st, fork, 0);
+        hipError_t err = fc_launch_relax_persistent(C.block, hb_batch ? 1 : 0, C.ngroups, C.lds, st, &B, &P, order,
+                                                    qinfo.data(), counters, C.cls, C.nmax, C.mmax, C.wmax,
+                                                    sched_mode == 0 ? 1 : 0, pc_mode == 2 ? hslot + slot_off[k] : nullptr,
+                                                    C.h2lds);
+        if (!hip_ok(err, "relax_noalloc launch")) return -1;
+        hipEvent_t join = noalloc_events[1 + k];
+        hipEventRecord(join, st);
+        hipStreamWaitEvent(stream, join, 0);
+    }
+    d_nsteps.download(nsteps, stream);
+    d_status.download(status, stream);
+    if (!hip_ok(hipStreamSynchronize(stream), "relax_noalloc")) return -1;
     int mx = 0;
     for (int s = 0; s < nSys; s++) mx = std::max(mx, nsteps[s]);
     return mx;

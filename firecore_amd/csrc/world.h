@@ -190,10 +190,19 @@ struct World {
     void register_buffers();
     fcd::BatchDev batch_dev();
     fcd::RunParams run_params(double Fconv, int nstep_max);
-    void plan_classes();
+    // launch plan of the persistent kernel; upload = false leaves the queue
+    // order in h_order only (the allocation-free path copies it itself)
+    void plan_classes(bool upload = true);
+    std::vector<int> h_order;
     // device entry points
     int relax_persistent(int nstep_max, double Fconv, bool restore_start);
     int relax_split(int nstep_max, double Fconv, bool restore_start);
+    // relax_persistent with every scratch buffer in the library's static device
+    // array (no allocation); -1 with a message when the batch needs more scratch
+    // than FC_NOALLOC_SCRATCH_BYTES or a path that allocates
+    int relax_noalloc(int nstep_max, double Fconv, bool restore_start);
+    void* noalloc_ptr = nullptr;  // this device's scratch array (looked up at set-up)
+    std::vector<hipEvent_t> noalloc_events;  // fork/join events of the class launches, made at set-up
     bool relax_unclassed(int nstep_max, double Fconv);
     // screening caller (MolWorld_sp3_multi::scan_relaxed): every configuration
     // a system of one batch, relaxed together; the previous batch is restored

@@ -44,6 +44,8 @@ def lib():
                                c_double_p, c_double_p, c_double_p, c_int_p]
     L.init_systems.restype = c_int
     L.relax.argtypes = [c_int, c_double, c_int, c_bool, c_bool]
+    L.relax_noalloc.argtypes = [c_int, c_double, c_bool, c_bool]
+    L.relax_noalloc.restype = c_int
     L.relax.restype = c_int
     L.get_last_ms.restype = c_double
     L.eval_forces.restype = c_int
@@ -311,6 +313,14 @@ def relax(nstep_max, Fconv, path=0, restore_start=False, download=True):
     n = lib().relax(nstep_max, Fconv, path, restore_start, download)
     if n < 0:
         raise RuntimeError("relax failed")
+    return n
+
+
+def relax_noalloc(nstep_max, Fconv, restore_start=False, download=True):
+    """relax() on the persistent kernel with no device allocation (static scratch)."""
+    n = lib().relax_noalloc(nstep_max, Fconv, restore_start, download)
+    if n < 0:
+        raise RuntimeError("relax_noalloc failed")
     return n
 
 

@@ -200,6 +200,18 @@ int gen_workload_arrays(int kind, int i, long long seed0, int nmin, int nmax, do
  * restore_start restarts from the last uploaded positions; bDownload copies
  * the state back to the host mirrors.  Returns max steps, -1 on error. */
 int relax(int nstepMax, double Fconv, int path, bool restore_start, bool bDownload);
+/* relax() on the persistent kernel (path 0) without any device allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The relaxation's scratch (queue counters and order, activation
+ * slots of the in-kernel network, GridFF window cache: 256 B per atom) lives
+ * in a static device array of 64 MiB per device, held until the process ends.
+ * Returns -1 with a message when the batch needs more scratch (about 250,000
+ * atoms) or a path that allocates (systems above 256 atoms, asymmetric
+ * exclusions, the chunked learned form, per-system timing).  Calls that
+ * overlap on one device share its scratch and race.  Results are those of
+ * relax(path 0). */
+int relax_noalloc(int nstepMax, double Fconv, bool restore_start, bool bDownload);
 /* Device time of the last relax (HIP events), ms. */
 double get_last_ms(void);
 /* Queue order of the persistent kernel (longest first) by the estimated
