@@ -225,8 +225,14 @@ def init(nSys_=10, xyz_name="input.xyz", surf_name=None, smile_name=None, sEleme
          sAtomTypes="data/AtomTypes.dat", sBondTypes="data/BondTypes.dat", sAngleTypes="data/AngleTypes.dat",
          bMMFF=True, bEpairs=False, nPBC=(1, 1, 0), gridnPBC=(1, 1, 0), gridStep=0.1, T=-1, gamma=-1, nExplore=0,
          nRelax=0, pos_kick=0.0, vel_kick=0.0, GridFF=5):
-    """xyz_name 'synth:natoms=N,seed=S,sigma=X' builds nSys_ replicas of a
-    synthetic molecule; periodic molecules are not supported (use nPBC=(0,0,0))."""
+    """nSys_ replicas of the typed molecule of xyz_name (".xyz" appended when
+    missing; periodic with nPBC when the file has an lvs line), typed with
+    common_resources/*.dat of the working directory when present, else with
+    the given sElementTypes/sAtomTypes/sBondTypes.  surf_name with GridFF > 0
+    adds the substrate's GridFF, with GridFF <= 0 its atoms (images gridnPBC).
+    xyz_name 'synth:natoms=N,seed=S,sigma=X' builds nSys_ replicas of a
+    synthetic molecule instead.  bEpairs (explicit electron pairs) is not
+    supported; a molecule the reference refuses raises RuntimeError."""
     global glob_bMMFF, nSys, isInitialized
     nSys = nSys_
     glob_bMMFF = bMMFF
@@ -548,6 +554,12 @@ def scan(poss, rots=None, Es=None, aforces=None, aposs=None, bF=False, bP=False,
          niter_max=10000, dt=0.05, Fconv=1e-5, Flim=100.0):
     nconf = len(poss)
     Es = np.zeros(nconf) if Es is None else Es
+    # outputs allocated on request, one (natoms, 3) block per configuration
+    # (pyBall/MMFF_multi.py:592-599)
+    if aforces is None and bF:
+        aforces = np.zeros((nconf, natoms, 3))
+    if aposs is None and bP:
+        aposs = np.zeros((nconf, natoms, 3))
     lib.scan(nconf, _np_as(poss, c_double_p), _np_as(rots, c_double_p), _np_as(Es, c_double_p),
              _np_as(aforces, c_double_p), _np_as(aposs, c_double_p), omp, bRelax, niter_max, dt, Fconv, Flim)
     return Es, aforces, aposs

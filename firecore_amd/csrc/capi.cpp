@@ -110,6 +110,23 @@ bool setup_surface(const char* surf_name, double gridStep, fc::GridFFHost& g) {
     return true;
 }
 
+// Substrate atoms of surf_name ("synth:NaCl" or an .xyz with an lvs line),
+// MolWorld_sp3::loadSurf with bGrid = false (MolWorld_sp3.h:1058-1084).
+bool load_surface_atoms(const char* surf_name, fc::Surface& S) {
+    std::string sn = surf_name;
+    if (sn.rfind("synth:NaCl", 0) == 0) {
+        fc::make_nacl_slab(5.64, 2, 0.9, S);
+        return true;
+    }
+    std::string fx = sn;
+    if (fx.size() < 4 || fx.substr(fx.size() - 4) != ".xyz") fx += ".xyz";
+    if (!fc::load_surface_xyz(fx, S)) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] cannot read surface %s\n", fx.c_str());
+        return false;
+    }
+    return true;
+}
+
 bool finish_init(World& W) {
     if (!W.ensure_device()) return false;
     if (!W.upload_all(true, true, true)) return false;
@@ -165,37 +182,72 @@ int* getIBuff(const char* name) {
 // ========================================================= MMFFmulti_lib.cpp
 void init_buffers() { world().register_buffers(); }
 
-// xyz_name: "synth:natoms=<N>,seed=<s>,sigma=<A>" builds the template molecule
-// with the synthetic generator and nSys replicas, replica i perturbed with seed
-// s+1+i (the builder path from a typed .xyz is not part of this library).
+// xyz_name: a typed molecule file (".xyz" appended when missing), built as the
+// reference's MolWorld_sp3::init does with bMMFF (MolWorld_sp3.h:309-357;
+// typing tables from common_resources/ in the working directory as the
+// reference loads them, else from the given sElementTypes/sAtomTypes/
+// sBondTypes paths): nSys identical replicas, periodic with init's nPBC when
+// the file has an lvs line.  "synth:natoms=<N>,seed=<s>,sigma=<A>" builds the
+// template with the synthetic generator instead, replica i perturbed with
+// seed s+1+i.  bEpairs (explicit electron pairs) is not supported.
 void* init(int nSys, char* xyz_name, char* surf_name, char* smile_name, bool bMMFF, bool bEpairs, int* nPBC,
            int* grid_nPBC, double gridStep, char* sElementTypes, char* sAtomTypes, char* sBondTypes,
            char* sAngleTypes, double T, double gamma, int nExplore, int nRelax, double pos_kick, double vel_kick,
            int GridFF) {
-    (void)smile_name; (void)bEpairs; (void)grid_nPBC; (void)sElementTypes; (void)sAtomTypes; (void)sBondTypes;
-    (void)sAngleTypes; (void)T; (void)gamma; (void)nExplore; (void)nRelax; (void)pos_kick; (void)vel_kick;
+    (void)smile_name; (void)sAngleTypes; (void)pos_kick; (void)vel_kick;
     World& W = world();
-    if (nPBC && (nPBC[0] || nPBC[1] || nPBC[2]))
-        std::fprintf(stderr, "[libMMFFmulti_lib] periodic molecules (nPBC>0) are not supported; using nPBC=0\n");
-    if (!xyz_name || std::strncmp(xyz_name, "synth:", 6) != 0) {
-        std::fprintf(stderr,
-                     "[libMMFFmulti_lib] init(%s): typed-topology building from .xyz is not part of this library; "
-                     "use xyz_name='synth:natoms=N,seed=S,sigma=X' or init_systems()/init_workload()\n",
-                     xyz_name ? xyz_name : "(null)");
+    // global-optimisation parameters (W.go, MMFFmulti_lib.cpp:91-96): run() explores when nExplore > 0
+    W.go_T = T;
+    W.go_gamma = gamma;
+    W.go_nExplore = nExplore;
+    W.go_nRelax = nRelax;
+    for (int k = 0; k < 3; k++) W.nPBC_init[k] = nPBC ? std::max(nPBC[k], 0) : 0;
+    if (!xyz_name || nSys < 1) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] init: needs xyz_name and nSys >= 1\n");
         return nullptr;
     }
-    int natoms = 24;
-    long long seed = 0;
-    double sigma = 0.1;
-    {
-        std::string spec(xyz_name + 6);
+    fc::MolSys tmpl;
+    std::vector<fc::MolSys> ms;
+    if (std::strncmp(xyz_name, "synth:", 6) != 0) {
+        if (bEpairs) std::fprintf(stderr, "[libMMFFmulti_lib] init: explicit electron pairs are not supported; building without them\n");
+        auto pick = [](const char* given, const char* deflt) {
+            if (file_exists(deflt) || !given) return std::string(deflt);
+            return std::string(given);
+        };
+        fc::TypeTables tables;
+        std::string err;
+        if (!tables.load(pick(sElementTypes, "common_resources/ElementTypes.dat"),
+                         pick(sAtomTypes, "common_resources/AtomTypes.dat"),
+                         pick(sBondTypes, "common_resources/BondTypes.dat"), err)) {
+            std::fprintf(stderr, "[libMMFFmulti_lib] init: %s\n", err.c_str());
+            return nullptr;
+        }
+        std::string fn = xyz_name;
+        if (fn.size() < 4 || fn.substr(fn.size() - 4) != ".xyz") fn += ".xyz";
+        bool periodic = false;
+        double lvec[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (!fc::build_from_xyz(fn, tables, periodic, lvec, tmpl, err)) {
+            std::fprintf(stderr, "[libMMFFmulti_lib] init(%s): %s\n", fn.c_str(), err.c_str());
+            return nullptr;
+        }
+        if (periodic) {  // MolWorld_sp3::makeMMFFs PBC set-up with init's nPBC
+            tmpl.pbc = true;
+            std::memcpy(tmpl.lvec, lvec, sizeof(lvec));
+            for (int k = 0; k < 3; k++) tmpl.nPBC[k] = W.nPBC_init[k];
+        }
+        ms.assign(nSys, tmpl);
+    } else {
+        int natoms = 24;
+        long long seed = 0;
+        double sigma = 0.1;
+        const std::string spec(xyz_name + 6);
         size_t p = 0;
         while (p < spec.size()) {
-            size_t q = spec.find(',', p);
-            std::string kv = spec.substr(p, q == std::string::npos ? std::string::npos : q - p);
-            size_t e = kv.find('=');
+            const size_t q = spec.find(',', p);
+            const std::string kv = spec.substr(p, q == std::string::npos ? std::string::npos : q - p);
+            const size_t e = kv.find('=');
             if (e != std::string::npos) {
-                std::string k = kv.substr(0, e), v = kv.substr(e + 1);
+                const std::string k = kv.substr(0, e), v = kv.substr(e + 1);
                 if (k == "natoms") natoms = std::atoi(v.c_str());
                 else if (k == "seed") seed = std::atoll(v.c_str());
                 else if (k == "sigma") sigma = std::atof(v.c_str());
@@ -203,14 +255,13 @@ void* init(int nSys, char* xyz_name, char* surf_name, char* smile_name, bool bMM
             if (q == std::string::npos) break;
             p = q + 1;
         }
+        if (!fc::gen_molecule(natoms, (uint64_t)seed, tmpl)) {
+            std::fprintf(stderr, "[libMMFFmulti_lib] init: cannot generate a %d-atom molecule\n", natoms);
+            return nullptr;
+        }
+        ms.assign(nSys, tmpl);
+        for (int i = 0; i < nSys; i++) fc::perturb_positions(ms[i], (uint64_t)(seed + 1 + i), sigma);
     }
-    fc::MolSys tmpl;
-    if (!fc::gen_molecule(natoms, (uint64_t)seed, tmpl)) {
-        std::fprintf(stderr, "[libMMFFmulti_lib] init: cannot generate a %d-atom molecule\n", natoms);
-        return nullptr;
-    }
-    std::vector<fc::MolSys> ms(nSys, tmpl);
-    for (int i = 0; i < nSys; i++) fc::perturb_positions(ms[i], (uint64_t)(seed + 1 + i), sigma);
     W.templ = tmpl;
     W.has_templ = true;
     W.bMMFF = bMMFF;
@@ -222,6 +273,11 @@ void* init(int nSys, char* xyz_name, char* surf_name, char* smile_name, bool bMM
         W.set_grid(g);
     }
     if (!finish_init(W)) return nullptr;
+    if (surf_name && GridFF <= 0) {  // substrate as explicit atoms (bSurfAtoms without bGridFF)
+        fc::Surface S;
+        const int np0[3] = {1, 1, 0};
+        if (!load_surface_atoms(surf_name, S) || !W.set_surface_atoms(S, grid_nPBC ? grid_nPBC : np0)) return nullptr;
+    }
     return &W;
 }
 
@@ -233,7 +289,14 @@ int run(int nstepMax, double dt, double Fconv, int ialg, double* outE, double* o
         std::fprintf(stderr, "[libMMFFmulti_lib] run: not initialised\n");
         return -1;
     }
-    int n = (iParalel == 2) ? W.relax_split(nstepMax, Fconv, false) : W.relax_persistent(nstepMax, Fconv, false);
+    int n;
+    if (W.go_nExplore > 0) {  // exploring requested at init: relax-explore cycles on the split kernels
+        n = W.explore(nstepMax, Fconv, W.go_T, W.go_gamma, W.go_nExplore, W.go_nRelax, 0x5EEDu, 16, true) < 0
+                ? -1
+                : nstepMax;
+    } else {
+        n = (iParalel == 2) ? W.relax_split(nstepMax, Fconv, false) : W.relax_persistent(nstepMax, Fconv, false);
+    }
     if (n < 0) return n;
     // state stays on the device for the next call: refresh the start copy
     hipMemcpyAsync(W.d_atoms0.p, W.d_atoms.p, sizeof(float4) * W.ntotV, hipMemcpyDeviceToDevice, W.stream);
@@ -297,9 +360,25 @@ void upload(bool bParams, bool bForces, bool bVel, bool blvec) {
 }
 void download(bool bForces, bool bVel) { world().download_all(bForces, bVel); }
 
+// MolWorld_sp3_multi::change_lvec / add_to_lvec (MolWorld_sp3_multi.h:1167-1200):
+// bAdd = false sets the lattice of the current system (system 0), bAdd = true
+// adds dlvec to the lattice of every periodic system; the image shifts follow.
+// Systems become periodic with the image range of init's nPBC.
 void change_lvec(double* lvec, bool bAdd, bool bUpdatePi) {
-    (void)lvec; (void)bAdd; (void)bUpdatePi;
-    unsupported("change_lvec");
+    (void)bUpdatePi;
+    World& W = world();
+    if (!lvec || W.nSys == 0) return;
+    if (!bAdd) {
+        if (!W.set_lattice(0, lvec, W.mols[0].pbc ? W.mols[0].nPBC : W.nPBC_init))
+            std::fprintf(stderr, "[libMMFFmulti_lib] change_lvec failed\n");
+        return;
+    }
+    for (int s = 0; s < W.nSys; s++) {
+        if (!W.mols[s].pbc) continue;
+        double L[9];
+        for (int k = 0; k < 9; k++) L[k] = W.mols[s].lvec[k] + lvec[k];
+        if (!W.set_lattice(s, L, W.mols[s].nPBC)) std::fprintf(stderr, "[libMMFFmulti_lib] change_lvec failed\n");
+    }
 }
 void optimizeLattice_1d(double* dlvec, int n1, int n2, int initMode, double tol) {
     (void)dlvec; (void)n1; (void)n2; (void)initMode; (void)tol;
@@ -775,6 +854,37 @@ int attach_surface(const char* surf_name, double gridStep) {
     return W.set_grid(g) ? 0 : -1;
 }
 
+// The substrate of surf_name as explicit atoms with images -nPBC..nPBC of its
+// lattice (GridFF::evalMorsePBC, the reference's CPU path without a grid);
+// replaces the GridFF.  surf_name = null removes the substrate atoms.
+int attach_surface_atoms(const char* surf_name, const int* nPBC3) {
+    World& W = world();
+    if (W.nSys == 0) return -1;
+    fc::Surface S;
+    if (surf_name && !load_surface_atoms(surf_name, S)) return -1;
+    const int np0[3] = {1, 1, 0};
+    return W.set_surface_atoms(S, nPBC3 ? nPBC3 : np0) ? 0 : -1;
+}
+
+// Substrate atoms (double): those of surf_name read on the host (no device
+// work; nPBC3 untouched), or with surf_name = null the ones in use.  Positions
+// (n x 3), REQs (n x 4), lattice rows (9) and image range (3), at most maxn
+// atoms.  Returns their number, -1 if surf_name cannot be read.
+int get_surface_atoms(const char* surf_name, double* apos, double* REQs, double* lvec9, int* nPBC3, int maxn) {
+    const World& W = world();
+    fc::Surface loaded;
+    if (surf_name && !load_surface_atoms(surf_name, loaded)) return -1;
+    const fc::Surface& S = surf_name ? loaded : W.surf_host;
+    const int n = (surf_name || !W.surf_pos.empty()) ? S.natoms() : 0;
+    const int m = std::min(n, std::max(maxn, 0));
+    if (apos) std::memcpy(apos, S.apos.data(), sizeof(double) * 3 * m);
+    if (REQs) std::memcpy(REQs, S.REQs.data(), sizeof(double) * 4 * m);
+    if (lvec9) std::memcpy(lvec9, S.lvec, sizeof(double) * 9);
+    if (nPBC3 && !surf_name)
+        for (int k = 0; k < 3; k++) nPBC3[k] = W.surf_nPBC[k];
+    return n;
+}
+
 // Relax the whole batch on the device.
 //   path 0: persistent kernel; 1: split-phase kernels
 //   restore_start: restart from the positions of the last init/upload
@@ -799,6 +909,59 @@ int relax_noalloc(int nstepMax, double Fconv, bool restore_start, bool bDownload
     int n = W.relax_noalloc(nstepMax, Fconv, restore_start);
     if (n >= 0 && bDownload && !W.download_all(true, true)) return -1;
     return n;
+}
+
+// Global-optimisation exploring (MolWorld_sp3_multi::updateMultiExploring,
+// MolWorld_sp3_multi.h:770-833, with the Langevin drive of updateAtomsMMFFf4,
+// relax_multi.cl:1160-1178): every system alternates FIRE relaxation and
+// Langevin exploring (nExplore steps at temperature T, damping gamma) for
+// nstepMax steps from its current state, recording up to kmax minima (energy,
+// and positions when bPositions).  Returns the number of minima found over all
+// systems, -1 on error.
+int explore(int nstepMax, double Fconv, double T, double gamma, int nExplore, int nRelax, long long seed, int kmax,
+            bool bPositions) {
+    World& W = world();
+    if (!W.dev_ready || W.nSys == 0) return -1;
+    int n = W.explore(nstepMax, Fconv, T, gamma, nExplore, nRelax, (unsigned)seed, kmax, bPositions);
+    if (n >= 0) W.download_all(true, true);
+    return n;
+}
+
+// Minima recorded for system isys by the last explore(): energies into Es and,
+// if recorded and poss is given, positions (natoms+nnode rows of x,y,z per
+// minimum) into poss; at most maxn.  Returns the number of minima found (may
+// exceed the number recorded), -1 on error.
+int get_minima(int isys, double* Es, double* poss, int maxn) {
+    World& W = world();
+    if (isys < 0 || isys >= W.nSys || W.nmin.size() != (size_t)W.nSys) return -1;
+    const int found = W.nmin[isys];
+    const int nrec = std::min(std::min(found, W.xkmax), std::max(maxn, 0));
+    const fcd::SysInfo& I = W.info[isys];
+    const int V = I.natoms + I.nnode;
+    for (int m = 0; m < nrec; m++) {
+        if (Es) Es[m] = W.minE[(size_t)isys * W.xkmax + m];
+        if (poss && W.minpos_kept)
+            for (int v = 0; v < V; v++) {
+                const float4 p = W.minpos[(size_t)m * W.ntotV + I.v0 + v];
+                poss[((size_t)m * V + v) * 3] = p.x;
+                poss[((size_t)m * V + v) * 3 + 1] = p.y;
+                poss[((size_t)m * V + v) * 3 + 2] = p.z;
+            }
+    }
+    return found;
+}
+
+// Periodic molecules (MMFFsp3_loc with bPBC: image sums of the non-bonded
+// terms, bonded neighbours at their nearest image, MMFFsp3_loc.h:325-329,
+// NBFF.h:575-620): lattice rows lvec9 (a, b, c) and image range nPBC3 (null =
+// keep the system's, or init's) of system isys, or of all systems if isys < 0;
+// lvec9 = null makes them vacuum molecules again.  Periodic systems are relaxed
+// by the split kernels.  0 on success, -1 on error.
+int set_lattice(int isys, const double* lvec9, const int* nPBC3) {
+    World& W = world();
+    if (W.nSys == 0 || isys >= W.nSys) return -1;
+    const int* np = nPBC3 ? nPBC3 : W.nPBC_init;
+    return W.set_lattice(isys, lvec9, np) ? 0 : -1;
 }
 
 double get_last_ms() { return world().last_ms; }
@@ -1133,6 +1296,54 @@ int gen_workload_arrays(int kind, int i, long long seed0, int nmin, int nmax, do
     if (Kpp) std::memcpy(Kpp, m.Kpp.data(), m.Kpp.size() * 8);
     if (elem) std::memcpy(elem, m.elem.data(), m.elem.size() * 4);
     return m.natoms;
+}
+// Typed molecule of an .xyz file on the host (the builder of init(xyz_name)):
+// tables from the given ElementTypes/AtomTypes/BondTypes files.  Outputs in the
+// MMFFsp3_loc layout (apos (natoms+nnode) x 3 with the pi-vectors, REQs x 4,
+// neighs x 4, apars/bLs/bKs/Ksp/Kpp nnode x 4, elem), the lattice and the
+// periodic flag.  Returns natoms, -1 on error (message on stderr), -2 if the
+// molecule has more than maxn atoms.
+int build_xyz_arrays(const char* xyz, const char* fElem, const char* fAtom, const char* fBond, int maxn,
+                     int* nnode_out, double* apos, double* REQs, int* neighs, double* apars, double* bLs, double* bKs,
+                     double* Ksp, double* Kpp, int* elem, double* lvec9, int* periodic) {
+    fc::TypeTables T;
+    std::string err;
+    if (!T.load(fElem, fAtom, fBond, err)) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] build_xyz_arrays: %s\n", err.c_str());
+        return -1;
+    }
+    fc::MolSys m;
+    bool pbc = false;
+    double lv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!fc::build_from_xyz(xyz, T, pbc, lv, m, err)) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] build_xyz_arrays(%s): %s\n", xyz, err.c_str());
+        return -1;
+    }
+    if (m.natoms > maxn) return -2;
+    *nnode_out = m.nnode;
+    std::memcpy(apos, m.apos.data(), m.apos.size() * 8);
+    std::memcpy(REQs, m.REQs.data(), m.REQs.size() * 8);
+    std::memcpy(neighs, m.neighs.data(), m.neighs.size() * 4);
+    std::memcpy(apars, m.apars.data(), m.apars.size() * 8);
+    std::memcpy(bLs, m.bLs.data(), m.bLs.size() * 8);
+    std::memcpy(bKs, m.bKs.data(), m.bKs.size() * 8);
+    std::memcpy(Ksp, m.Ksp.data(), m.Ksp.size() * 8);
+    std::memcpy(Kpp, m.Kpp.data(), m.Kpp.size() * 8);
+    std::memcpy(elem, m.elem.data(), m.elem.size() * 4);
+    std::memcpy(lvec9, lv, sizeof(lv));
+    *periodic = pbc ? 1 : 0;
+    return m.natoms;
+}
+
+// Image shifts (evalPBCshifts order) and bonded-neighbour images
+// (makeNeighCells) of a periodic molecule; returns npbc.
+int make_neigh_cells(int natoms, const double* apos, const int* neighs, const double* lvec9, const int* nPBC3,
+                     double* shifts_out, int* neighCell_out) {
+    std::vector<double> sh;
+    const int npbc = fc::make_pbc_shifts(lvec9, nPBC3, sh);
+    if (shifts_out) std::memcpy(shifts_out, sh.data(), sh.size() * sizeof(double));
+    if (neighCell_out) fc::make_neigh_cells(natoms, apos, neighs, npbc, sh.data(), neighCell_out);
+    return npbc;
 }
 int make_back_neighs(int natoms, int nnode, const int* neighs, int* bkneighs) {
     return fc::make_back_neighs(natoms, nnode, neighs, bkneighs) ? 0 : -1;

@@ -12,7 +12,9 @@
 //   bLs, bKs, Ksp, Kpp: 4 per node (one per neighbour slot)
 #pragma once
 #include <cstdint>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 namespace fc {
@@ -33,6 +35,11 @@ struct MolSys {
     std::vector<int> elem;      // natoms (EL_*)
     std::vector<double> fapos;  // (natoms+nnode)*3, filled by unpack with forces
     std::vector<double> vapos;  // (natoms+nnode)*3, filled by unpack with velocities
+    // periodic molecule (MMFFsp3_loc with bPBC): lattice vectors as rows a, b, c
+    // and the image range -nPBC..nPBC per axis; vacuum when pbc is false
+    bool pbc = false;
+    double lvec[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int nPBC[3] = {0, 0, 0};
     int nvecs() const { return natoms + nnode; }
 };
 
@@ -93,7 +100,51 @@ void make_exclusions(int natoms, int nnode, const int* neighs, bool subtract13,
 // Rotated rows (bit k of row i = pair (i, (i+k) mod N)) for the half-ring
 // schedule of the persistent kernel; false if the exclusions are asymmetric.
 int connected_components(int natoms, const int* neighs, int maxc, int* comp);
+// Periodic molecules: image shifts (MolWorld_sp3::evalPBCshifts order) and the
+// image of each bonded neighbour (MMFFsp3_loc::makeNeighCells(npbc, shifts)).
+int make_pbc_shifts(const double* lvec, const int* nPBC, std::vector<double>& shifts);
+void make_neigh_cells(int natoms, const double* apos, const int* neighs, int npbc, const double* shifts,
+                      int* neighCell);
 bool rotate_exclusions(int natoms, const std::vector<uint32_t>& mask, int W, std::vector<uint32_t>& rot);
+
+// ---------------- typed molecules from .xyz (builder.cpp) ----------------
+// Parameter tables of the reference (ElementTypes.dat, AtomTypes.dat,
+// BondTypes.dat; MMFFparams.h:59-130, 196-384).
+struct ElementTypeB {
+    std::string name;
+    int iZ = 0, neval = 0, valence = 0, piMax = 0;
+    double Rcov = 0, RvdW = 0, EvdW = 0;
+};
+struct AtomTypeB {
+    std::string name;
+    int iZ = 0;
+    uint8_t valence = 0, nepair = 0, npi = 0, sym = 0;
+    double Ruff = 0, RvdW = 0, EvdW = 0, Qbase = 0, Hb = 0;
+    int parent = 0, element = 0, ePairType = 0;
+    bool bMMFF = false;
+    double Ass = 0, Asp = 0, Kss = 0, Ksp = 0, Kep = 0, Kpp = 0;
+    int subTypes[3] = {0, 0, 0};  // the type's _3, _2, _1 variants (0 when absent)
+};
+struct BondTypeB {
+    double l0, k;
+};
+struct TypeTables {
+    std::vector<ElementTypeB> etypes;
+    std::unordered_map<std::string, int> edict;
+    std::vector<AtomTypeB> atypes;
+    std::unordered_map<std::string, int> adict;
+    std::map<uint64_t, BondTypeB> bonds;
+    bool load(const std::string& fElem, const std::string& fAtom, const std::string& fBond, std::string& err);
+    int atom_type(const std::string& name) const;  // -1 if unknown
+    static uint64_t bond_key(int a, int b, int order);
+    void bond_params(int ti, int tj, int order, double& l0, double& k) const;
+};
+// The reference's MMFF build of an .xyz molecule (MolWorld_sp3::init with
+// bMMFF, bEpairs = false): typed atoms, bonds, MMFFsp3_loc parameters and
+// pi-vectors, nodes first.  periodic/lvec9 report an lvs line.  False with a
+// message in err on a file or typing error.
+bool build_from_xyz(const std::string& fname, const TypeTables& T, bool& periodic, double* lvec9, MolSys& out,
+                    std::string& err);
 
 // ---------------- GridFF (gridff_host.cpp) ----------------
 struct GridFFHost {

@@ -99,11 +99,13 @@ __device__ __forceinline__ float angle_cos_half(V3 h1, V3 h2, float ir1, float i
 // make_exclusions), which gives the same total force and energy.
 // Outputs: recoils on the 4 neighbours (fbs) and on their pi-vectors (fps),
 // the node's own force (fa) and its pi force (fpi).  Returns the energy.
-// `pos(i)` returns the position of vec i (atom or, for i>=natoms, pi-vector).
-template <class PosFn>
-__device__ __forceinline__ float eval_node(int ia, int natoms, int nnode, PosFn pos, const int4 ng, const float4 apar,
-                                           const float4 bL, const float4 bK, const float4 ksp, const float4 kpp,
-                                           const FFSwitch sw, V3* fbs, V3* fps, V3& fa, V3& fpi) {
+// `pos(i)` returns the position of vec i (atom or, for i>=natoms, pi-vector);
+// `nshift(k)` the periodic image shift of neighbour slot k (pbc_shifts of its
+// neighCell, MMFFsp3_loc.h:325-329; zero in vacuum).
+template <class PosFn, class ShiftFn>
+__device__ __forceinline__ float eval_node(int ia, int natoms, int nnode, PosFn pos, ShiftFn nshift, const int4 ng,
+                                           const float4 apar, const float4 bL, const float4 bK, const float4 ksp,
+                                           const float4 kpp, const FFSwitch sw, V3* fbs, V3* fps, V3& fa, V3& fpi) {
     const V3 pa = pos(ia);
     const V3 hpi = pos(natoms + ia);
     const int ings[4] = {ng.x, ng.y, ng.z, ng.w};
@@ -128,7 +130,7 @@ __device__ __forceinline__ float eval_node(int ia, int natoms, int nnode, PosFn 
         const int ing = ings[i];
         if (ing < 0) break;
         nb = i + 1;
-        V3 h = pos(ing) - pa;
+        V3 h = pos(ing) + nshift(i) - pa;
         const float l = sqrtf(dot(h, h));
         const float il = 1.f / l;
         h = h * il;
@@ -176,6 +178,14 @@ __device__ __forceinline__ float eval_node(int ia, int natoms, int nnode, PosFn 
         }
     }
     return E;
+}
+// Vacuum molecules: no image shifts.
+template <class PosFn>
+__device__ __forceinline__ float eval_node(int ia, int natoms, int nnode, PosFn pos, const int4 ng, const float4 apar,
+                                           const float4 bL, const float4 bK, const float4 ksp, const float4 kpp,
+                                           const FFSwitch sw, V3* fbs, V3* fps, V3& fa, V3& fpi) {
+    return eval_node(ia, natoms, nnode, pos, [](int) { return zero3(); }, ng, apar, bL, bK, ksp, kpp, sw, fbs, fps, fa,
+                     fpi);
 }
 
 // -------------------------------------------------------------------------
@@ -457,6 +467,35 @@ __device__ __forceinline__ void fire_update(const FireParams& P, FireState& S, f
         S.lastNeg++;
     }
     cfe = renorm * cf;
+}
+
+// -------------------------------------------------------------------------
+// Langevin drive of the exploring phase (updateAtomsMMFFf4,
+// relax_multi.cl:1160-1178): f += -gamma*v + xi*sqrt(2*kB*T*gamma/dt) on atoms
+// (pi-vectors are not driven), with xi a unit-variance random number per
+// component.  xi comes from a counter-based hash of (seed, system, vec, axis,
+// step) -- murmur3's 32-bit finaliser applied after each field -- mapped to a
+// uniform on [-sqrt(3), sqrt(3)); integer arithmetic only, so the FP64 oracle
+// draws the same numbers.
+constexpr float BOLTZMANN_K = 8.617333262145e-5f;  // eV/K (cpp/common/constants.h:12)
+__host__ __device__ inline unsigned fmix32(unsigned h) {
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+__host__ __device__ inline unsigned drive_hash(unsigned seed, unsigned sys, unsigned vec, unsigned axis,
+                                               unsigned step) {
+    unsigned h = fmix32(seed + 0x9E3779B9u);
+    h = fmix32(h ^ sys);
+    h = fmix32(h ^ (vec * 3u + axis));
+    return fmix32(h ^ step);
+}
+__device__ __forceinline__ float drive_xi(unsigned seed, unsigned sys, unsigned vec, unsigned axis, unsigned step) {
+    const unsigned h = drive_hash(seed, sys, vec, axis, step) >> 8;  // 24 bits
+    return ((float)h * (2.f / 16777216.f) - 1.f) * 1.7320508f;
 }
 
 }  // namespace fcd

@@ -18,6 +18,7 @@ struct SysInfo {
     int v0;     // first vec    (atoms, avel, aforce, bkNeighs)
     int mask0;  // first word of the exclusion bit-matrix (natoms rows of W words)
     int corr0, ncorr;  // weighted pair corrections
+    int sh0, npbc;     // periodic molecule: its image shifts pbcshifts[sh0 .. sh0+npbc); npbc = 0 in vacuum
 };
 
 struct PairCorrDev {
@@ -85,6 +86,18 @@ struct BatchDev {
     // learned preconditioner gains (written by the proposal network)
     const float* gnode;    // [sum M] star gains
     const float* gcomp;    // [nsys * PC_MAXC] molecule gains
+    // periodic molecules: image shifts (xyz) of every system, indexed by SysInfo
+    // sh0/npbc, and the image of each bonded neighbour (neighCell, per atom)
+    const float4* pbcshifts;
+    const int4* neighCell;
+    // global-optimisation exploring (split path; null = off): per system
+    // xstate = (phase: 0 relaxing / 1 exploring, steps taken in the phase,
+    // minima found, -); minimum m of system s: energy minE[s * xkmax + m] and,
+    // when minpos is set, positions of the system's vecs at
+    // minpos[m * ntotV + v0 + v]
+    int4* xstate;
+    double* minE;
+    float4* minpos;
 };
 
 // Parameters of the proposal network (BF16 dense layers, FP32 head).
@@ -118,6 +131,24 @@ struct RunParams {
     // launch, resuming from (and suspending to) the state in B (positions,
     // avel, fire, nsteps, status); 0 = each system runs to completion
     int chunk;
+    // exploring (split path, MolWorld_sp3_multi::updateMultiExploring with the
+    // Langevin drive of updateAtomsMMFFf4): a system that converges records the
+    // minimum and is driven at temperature xT (K) with damping xgamma for
+    // xn_explore steps, then relaxes again; a relaxation phase longer than
+    // xn_relax steps (> 0) is abandoned the same way without a record
+    int explore;
+    float xT, xgamma;
+    int xn_explore, xn_relax;
+    unsigned xseed;
+    int xkmax;  // minima recorded per system (further ones are counted only)
+    // explicit substrate atoms instead of the GridFF (split path; nsurf = 0: none):
+    // positions, REQs (R0, sqrt(E0), Q, H) and the image shifts of the substrate
+    // lattice; Morse exponent surf_K = -alphaMorse
+    const float4* surf_pos;
+    const float4* surf_REQ;
+    const float4* surf_shifts;
+    int nsurf, nsurf_pbc;
+    float surf_K;
 };
 
 }  // namespace fcd

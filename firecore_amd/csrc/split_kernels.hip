@@ -11,6 +11,91 @@
 namespace fcd {
 
 // ---------------------------------------------------------------------------
+// Non-bonded force on atom i of a periodic molecule, NBFF::
+// evalLJQs_ng4_PBC_atom_omp (NBFF.h:575-620): every other atom j at every image
+// shift.  A pair the exclusion matrix marks (bonded or 1-3, net weight 0) is
+// left out at its nearest image only: the reference skips a bonded pair at the
+// image of the bond (neighCell) and subtracts a 1-3 pair at the image of its
+// angle's two bonds, which are the nearest images for molecules whose bonded
+// pairs are shorter than half the cell.  Weighted corrections (rings) act at
+// that image too.  Adds the force to f and the unhalved energy to e.
+__device__ void nb_pbc_atom(const BatchDev& B, const SysInfo& I, int i, V3 pi, float4 ri, V3& f, double& e) {
+    const int N = I.natoms, W = (N + 31) >> 5;
+    const float kQi = COULOMB_K * ri.z;
+    const float4* sh = B.pbcshifts + I.sh0;
+    auto nearest = [&](V3 d0) {
+        int smin = 0;
+        float r2min = INFINITY;
+        for (int s = 0; s < I.npbc; s++) {
+            const V3 d = d0 + xyz(sh[s]);
+            const float r2 = dot(d, d);
+            if (r2 < r2min) {
+                r2min = r2;
+                smin = s;
+            }
+        }
+        return smin;
+    };
+    for (int j = 0; j < N; j++) {
+        if (j == i) continue;
+        const V3 d0 = xyz(B.atoms[I.v0 + j]) - pi;
+        const float4 rj = B.REQs[I.a0 + j];
+        const bool ex = (B.excl[I.mask0 + i * W + (j >> 5)] >> (j & 31)) & 1u;
+        const int skip = ex ? nearest(d0) : -1;
+        for (int s = 0; s < I.npbc; s++) {
+            if (s == skip) continue;
+            const V3 d = d0 + xyz(sh[s]);
+            float fr, ep;
+            lj_pair<true, true>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+            f -= d * fr;
+            e += (double)ep;
+        }
+    }
+    for (int c = 0; c < I.ncorr; c++) {
+        const PairCorrDev pc = B.corr[I.corr0 + c];
+        if (pc.i != i) continue;
+        const V3 d0 = xyz(B.atoms[I.v0 + pc.j]) - pi;
+        const V3 d = d0 + xyz(sh[nearest(d0)]);
+        const float4 rj = B.REQs[I.a0 + pc.j];
+        float fr, ep;
+        lj_pair<true, true>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
+        f -= d * (fr * pc.w);
+        e += (double)ep * pc.w;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Explicit substrate atoms, GridFF::evalMorsePBC (GridFF.h:1229-1261) with the
+// pair term addAtomicForceMorseQ (Forces.h:540-561): for every substrate atom j
+// (REQs mixed by combineREQ: R0 sum, E0 and charge products) at every image
+// shift, dp = pi - pj - shift, r = |dp|, e = exp(K (r - R0)) with K =
+// -alphaMorse; E = E0 e (e - 2) + k q q / r and the force on i is
+// -dp (2 K E0 e (e - 1) / r - k q q / r^3).  Adds the force to f, returns E.
+__device__ float surf_atoms_force(const RunParams& P, V3 pi, float4 ri, V3& f) {
+    float E = 0.f;
+    const float K = P.surf_K;
+    for (int j = 0; j < P.nsurf; j++) {
+        const float4 a = P.surf_pos[j];
+        const float4 q = P.surf_REQ[j];
+        const float r0 = q.x + ri.x, e0 = q.y * ri.y, kqq = COULOMB_K * q.z * ri.z;
+        const V3 dp0 = mk(pi.x - a.x, pi.y - a.y, pi.z - a.z);
+        for (int s = 0; s < P.nsurf_pbc; s++) {
+            const V3 dp = dp0 + xyz(P.surf_shifts[s]);
+            const float r2 = dot(dp, dp);
+            const float ir = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * ir;
+            const float e = expf(K * (r - r0));
+            const float Ae = e0 * e;
+            const float eel = kqq * ir;
+            E += Ae * (e - 2.f) + eel;
+            const float fr = 2.f * K * Ae * (e - 1.f) * ir - eel * ir * ir;
+            f -= dp * fr;
+        }
+    }
+    return E;
+}
+
+// ---------------------------------------------------------------------------
 // Non-bonded (+GridFF) force per atom.  grid = (ceil(maxN/64), nsys), 64 threads.
 // aforce[v0+i] = (f, 0.5*E_nb + E_grid)   (assignment, like the reference NB kernel)
 __global__ void __launch_bounds__(64) k_nonbond_grid(BatchDev B, RunParams P) {
@@ -33,7 +118,10 @@ __global__ void __launch_bounds__(64) k_nonbond_grid(BatchDev B, RunParams P) {
     const float kQi = COULOMB_K * ri.z;
     V3 f = zero3();
     double e = 0.0;  // FP64 accumulation of the pair energies
-    if (P.bNonBonded) {
+    if (P.bNonBonded && I.npbc > 0) {  // periodic molecule (uniform over the block)
+        if (act) nb_pbc_atom(B, I, i, pi, ri, f, e);
+        e *= 0.5;
+    } else if (P.bNonBonded) {
         for (int j0 = 0; j0 < N; j0 += 64) {
             const int jl = j0 + threadIdx.x;
             __syncthreads();
@@ -83,6 +171,7 @@ __global__ void __launch_bounds__(64) k_nonbond_grid(BatchDev B, RunParams P) {
         const float4 plq = B.PLQs[I.a0 + i];
         e += (double)grid_sample(P.grid, pi, mk(plq.x, plq.y, plq.z), f);
     }
+    if (P.nsurf > 0) e += (double)surf_atoms_force(P, pi, ri, f);
     B.aforce[I.v0 + i] = make_float4(f.x, f.y, f.z, 0.f);
     B.enb[I.a0 + i] = e;
 }
@@ -102,9 +191,16 @@ __global__ void __launch_bounds__(256) k_bonded(BatchDev B, RunParams P, const i
     const int N = I.natoms, M = I.nnode;
     const float4* apos = B.atoms + I.v0;
     auto pos = [&](int v) { return xyz(apos[v]); };
+    // periodic molecules: neighbour k sits at its image pbc_shifts[neighCell[k]]
+    const int4 nc = I.npbc > 0 ? B.neighCell[I.a0 + k] : make_int4(-1, -1, -1, -1);
+    const float4* shifts = B.pbcshifts + I.sh0;
+    auto nshift = [&](int q) {
+        const int c = q == 0 ? nc.x : (q == 1 ? nc.y : (q == 2 ? nc.z : nc.w));
+        return c >= 0 ? xyz(shifts[c]) : zero3();
+    };
     V3 fbs[4], fps[4], fa, fpi;
-    float E = eval_node(k, N, M, pos, B.neighs[I.a0 + k], B.apars[g], B.bLs[g], B.bKs[g], B.Ksp[g], B.Kpp[g], P.sw,
-                        fbs, fps, fa, fpi);
+    float E = eval_node(k, N, M, pos, nshift, B.neighs[I.a0 + k], B.apars[g], B.bLs[g], B.bKs[g], B.Ksp[g], B.Kpp[g],
+                        P.sw, fbs, fps, fa, fpi);
     float4* fng = B.fneigh + (size_t)I.n0 * 8;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -157,6 +253,101 @@ __device__ __forceinline__ double blk_sum_d(double v, double* sh) {
 // ---------------------------------------------------------------------------
 // Assemble + per-system reduction + FIRE + move.  One block (256) per system.
 // do_move=0 only assembles (force/energy evaluation for parity checks).
+// Move of the exploring form: the per-system phase machine of
+// MolWorld_sp3_multi::updateMultiExploring (MolWorld_sp3_multi.h:770-833) run
+// on the device, with the sums and energy of this step from k_update.
+//   relaxing:  FIRE as in k_update; on convergence the minimum is recorded
+//              (energy at this geometry, positions when minpos is set) and the
+//              exploring phase starts; after xn_relax steps (> 0) it starts
+//              without a record;
+//   exploring: xn_explore steps of Langevin dynamics at dt_max with no FIRE
+//              mixing (MDpars = (dt_max, 1, 1, 0), evalVF_new :755-760; drive
+//              relax_multi.cl:1160-1178), then FIRE restarts from its initial
+//              state (dt_max, damp_max).
+// The run ends after nstep_max steps (status ST_MAXSTEP).
+__device__ void explore_move(const BatchDev& B, const RunParams& P, int s, const SysInfo& I, int step, float vf,
+                             float vv, float ff, float fm2, double e) {
+    const int N = I.natoms, V = I.natoms + I.nnode;
+    if (!(ff == ff) || isinf(ff)) {
+        if (threadIdx.x == 0) B.status[s] = ST_NAN;
+        return;
+    }
+    int4 xs = B.xstate[s];
+    float4 st = B.fire[s];
+    const float crit = (P.conv_mode == CONV_F2SUM) ? ff : fm2;
+    bool reset_fire = false;
+    if (xs.x == 0) {
+        const bool conv = crit < P.Fconv2;
+        if (conv || (P.xn_relax > 0 && xs.y >= P.xn_relax)) {
+            if (conv) {
+                const int m = xs.z;
+                if (m < P.xkmax) {
+                    if (threadIdx.x == 0) B.minE[(size_t)s * P.xkmax + m] = e;
+                    if (B.minpos)
+                        for (int v = threadIdx.x; v < V; v += blockDim.x)
+                            B.minpos[(size_t)m * B.ntotV + I.v0 + v] = B.atoms[I.v0 + v];
+                }
+                xs.z++;
+            }
+            xs.x = 1;
+            xs.y = 0;
+        }
+    } else if (xs.y >= P.xn_explore) {
+        xs.x = 0;
+        xs.y = 0;
+        reset_fire = true;
+    }
+    __syncthreads();  // every thread has read xstate and the FIRE state
+    if (step >= P.nstep_max) {
+        if (threadIdx.x == 0) {
+            B.status[s] = ST_MAXSTEP;
+            B.xstate[s] = xs;
+        }
+        return;
+    }
+    const bool drive = xs.x == 1;
+    float dt, cv, cfe;
+    if (drive) {
+        dt = P.fire.dt_max;
+        cv = 1.f;
+        cfe = 0.f;
+    } else {
+        FireState fs;
+        fs.dt = reset_fire ? P.fire.dt_max : st.x;
+        fs.damping = reset_fire ? P.fire.damp_max : st.y;
+        fs.lastNeg = reset_fire ? 0 : (int)st.z;
+        fire_update(P.fire, fs, vf, vv, ff, cv, cfe);
+        dt = fs.dt;
+        st = make_float4(fs.dt, fs.damping, (float)fs.lastNeg, 0.f);
+    }
+    if (threadIdx.x == 0) {
+        B.fire[s] = st;
+        xs.y++;
+        B.xstate[s] = xs;
+    }
+    const float amp = drive ? sqrtf(2.f * BOLTZMANN_K * P.xT * P.xgamma / dt) : 0.f;
+    for (int v = threadIdx.x; v < V; v += blockDim.x) {
+        const bool isPi = v >= N;
+        V3 p = xyz(B.atoms[I.v0 + v]);
+        V3 f = xyz(B.aforce[I.v0 + v]);
+        const V3 w0 = xyz(B.avel[I.v0 + v]);
+        if (isPi) {
+            f -= p * dot(p, f);
+        } else if (drive) {
+            f -= w0 * P.xgamma;
+            f.x += amp * drive_xi(P.xseed, (unsigned)s, (unsigned)v, 0u, (unsigned)step);
+            f.y += amp * drive_xi(P.xseed, (unsigned)s, (unsigned)v, 1u, (unsigned)step);
+            f.z += amp * drive_xi(P.xseed, (unsigned)s, (unsigned)v, 2u, (unsigned)step);
+        }
+        V3 w = w0 * cv + f * (dt + cfe);
+        if (isPi) w -= p * dot(p, w);
+        p += w * dt;
+        if (isPi) p = p * (1.f / sqrtf(dot(p, p)));
+        B.avel[I.v0 + v] = make_float4(w.x, w.y, w.z, 0.f);
+        B.atoms[I.v0 + v] = make_float4(p.x, p.y, p.z, 0.f);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_update(BatchDev B, RunParams P, int do_move, int step) {
     __shared__ float sh[8];
     __shared__ double shd[8];
@@ -210,6 +401,10 @@ __global__ void __launch_bounds__(256) k_update(BatchDev B, RunParams P, int do_
         B.nsteps[s] = step;
     }
     if (!do_move) return;
+    if (P.explore) {
+        explore_move(B, P, s, I, step, vf, vv, ff, fm2, e);
+        return;
+    }
     int status = ST_RUNNING;
     const float crit = (P.conv_mode == CONV_F2SUM) ? ff : fm2;
     if (!(ff == ff) || isinf(ff)) status = ST_NAN;

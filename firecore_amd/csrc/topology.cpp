@@ -194,4 +194,42 @@ void init_pi(int natoms, int nnode, const double* apos, const int* neighs, const
         }
 }
 
+// Periodic images (MolWorld_sp3::evalPBCshifts, MolWorld_sp3.h:756-765): shift
+// ipbc = ix*a + iy*b + iz*c for iz outer, iy, ix inner over -nPBC..nPBC.
+int make_pbc_shifts(const double* lvec, const int* nPBC, std::vector<double>& shifts) {
+    shifts.clear();
+    for (int iz = -nPBC[2]; iz <= nPBC[2]; iz++)
+        for (int iy = -nPBC[1]; iy <= nPBC[1]; iy++)
+            for (int ix = -nPBC[0]; ix <= nPBC[0]; ix++)
+                for (int q = 0; q < 3; q++) shifts.push_back(lvec[q] * ix + lvec[3 + q] * iy + lvec[6 + q] * iz);
+    return (int)(shifts.size() / 3);
+}
+
+// Image of each bonded neighbour (MMFFsp3_loc::makeNeighCells(npbc, shifts),
+// MMFFsp3_loc.h:1394-1434): the first image with the smallest |pj + shift -
+// pi|^2, compared in single precision as the reference does; -1 for empty slots.
+void make_neigh_cells(int natoms, const double* apos, const int* neighs, int npbc, const double* shifts,
+                      int* neighCell) {
+    for (int ia = 0; ia < natoms; ia++)
+        for (int j = 0; j < 4; j++) {
+            const int ja = neighs[ia * 4 + j];
+            neighCell[ia * 4 + j] = -1;
+            if (ja < 0) continue;
+            const double d[3] = {apos[ja * 3] - apos[ia * 3], apos[ja * 3 + 1] - apos[ia * 3 + 1],
+                                 apos[ja * 3 + 2] - apos[ia * 3 + 2]};
+            int imin = -1;
+            float r2min = INFINITY;
+            for (int ipbc = 0; ipbc < npbc; ipbc++) {
+                const double x = shifts[ipbc * 3] + d[0], y = shifts[ipbc * 3 + 1] + d[1],
+                             z = shifts[ipbc * 3 + 2] + d[2];
+                const float r2 = (float)(x * x + y * y + z * z);
+                if (r2 < r2min) {
+                    r2min = r2;
+                    imin = ipbc;
+                }
+            }
+            neighCell[ia * 4 + j] = imin;
+        }
+}
+
 }  // namespace fc

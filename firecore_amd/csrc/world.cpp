@@ -101,6 +101,54 @@ bool World::set_systems(std::vector<fc::MolSys>&& ms) {
     return true;
 }
 
+bool World::set_surface_atoms(const fc::Surface& s, const int* nPBC) {
+    surf_pos.clear();
+    surf_REQ.clear();
+    surf_shifts.clear();
+    surf_host = s;
+    for (int k = 0; k < 3; k++) surf_nPBC[k] = nPBC[k];
+    const int n = s.natoms();
+    for (int i = 0; i < n; i++) {
+        surf_pos.push_back(f4(s.apos[i * 3], s.apos[i * 3 + 1], s.apos[i * 3 + 2], 0.0));
+        surf_REQ.push_back(f4(s.REQs[i * 4], s.REQs[i * 4 + 1], s.REQs[i * 4 + 2], s.REQs[i * 4 + 3]));
+    }
+    if (n > 0) {  // GridFF::makePBCshifts(nPBC, lvec) (NBFF.h:1142-1156)
+        std::vector<double> sh;
+        fc::make_pbc_shifts(s.lvec, nPBC, sh);
+        for (size_t k = 0; k < sh.size() / 3; k++) surf_shifts.push_back(f4(sh[k * 3], sh[k * 3 + 1], sh[k * 3 + 2], 0.0));
+        bGridFF = false;  // the substrate enters through its atoms
+    }
+    if (!ensure_device()) return false;
+    d_surf_pos.upload(surf_pos, stream);
+    d_surf_REQ.upload(surf_REQ, stream);
+    d_surf_shifts.upload(surf_shifts, stream);
+    if (nSys > 0) plan_classes();
+    return hip_ok(hipStreamSynchronize(stream), "surface atoms");
+}
+
+bool World::set_lattice(int isys, const double* lvec, const int* nPBC) {
+    if (nSys == 0 || isys >= nSys) return false;
+    if (dev_ready) {  // keep the current state of every system
+        if (!download_all(true, true)) return false;
+        for (int s = 0; s < nSys; s++) unpack_system(s, true, true);
+    }
+    for (int s = (isys < 0 ? 0 : isys); s < (isys < 0 ? nSys : isys + 1); s++) {
+        fc::MolSys& m = mols[s];
+        m.pbc = lvec != nullptr;
+        if (lvec)
+            for (int k = 0; k < 9; k++) m.lvec[k] = lvec[k];
+        if (nPBC)
+            for (int k = 0; k < 3; k++) m.nPBC[k] = std::max(nPBC[k], 0);
+    }
+    const std::vector<float4> keep_constr = constr, keep_constrK = constrK, keep_bboxes = bboxes;
+    build_packed();
+    constr = keep_constr;
+    constrK = keep_constrK;
+    bboxes = keep_bboxes;
+    register_buffers();
+    return dev_ready ? upload_all(true, true, true) : true;
+}
+
 void World::rebuild_exclusions() {
     excl.clear();
     excl_rot.clear();
@@ -169,7 +217,18 @@ void World::build_packed() {
     ncomp.assign(nSys, 1);
     lvecs.assign((size_t)nSys * 3, f4(0, 0, 0, 0));
     ilvecs.assign((size_t)nSys * 3, f4(0, 0, 0, 0));
-    pbcshifts.assign((size_t)nSys, f4(0, 0, 0, 0));
+    // image shifts: npbc per periodic system, one zero shift per vacuum system
+    // (the reference's pbcshifts layout, nSys*npbc, for uniform batches)
+    int nsh = 0;
+    n_periodic = 0;
+    for (int s = 0; s < nSys; s++) {
+        const fc::MolSys& m = mols[s];
+        info[s].sh0 = nsh;
+        info[s].npbc = m.pbc ? (2 * m.nPBC[0] + 1) * (2 * m.nPBC[1] + 1) * (2 * m.nPBC[2] + 1) : 0;
+        nsh += std::max(info[s].npbc, 1);
+        n_periodic += m.pbc ? 1 : 0;
+    }
+    pbcshifts.assign((size_t)nsh, f4(0, 0, 0, 0));
     nsteps.assign(nSys, 0);
     status.assign(nSys, 0);
     Esys.assign(nSys, 0.0);
@@ -209,13 +268,39 @@ void World::pack_system(int s, bool bParams, bool bForces, bool bVel, bool blvec
         for (int v = 0; v < V; v++) aforces[I.v0 + v] = f4(m.fapos[v * 3], m.fapos[v * 3 + 1], m.fapos[v * 3 + 2], 0.0);
     if (bVel && (int)m.vapos.size() == V * 3)
         for (int v = 0; v < V; v++) avel[I.v0 + v] = f4(m.vapos[v * 3], m.vapos[v * 3 + 1], m.vapos[v * 3 + 2], 0.0);
+    std::vector<double> shifts;
+    if (m.pbc) fc::make_pbc_shifts(m.lvec, m.nPBC, shifts);
     if (blvec) {
-        // vacuum / non-periodic molecules: zero lattice, one zero PBC shift
+        // vacuum molecules: zero lattice, one zero shift; periodic ones: lattice
+        // rows, the transposed inverse (Mat3_to_cl(ff.invLvec), setLvec's
+        // invert_T_to, MMFFsp3_loc.h:232) and the image shifts
         for (int k = 0; k < 3; k++) {
             lvecs[s * 3 + k] = f4(0, 0, 0, 0);
             ilvecs[s * 3 + k] = f4(0, 0, 0, 0);
         }
-        pbcshifts[s] = f4(0, 0, 0, 0);
+        pbcshifts[I.sh0] = f4(0, 0, 0, 0);
+        if (m.pbc) {
+            const double* L = m.lvec;
+            double inv[9];
+            const double det = L[0] * (L[4] * L[8] - L[5] * L[7]) - L[1] * (L[3] * L[8] - L[5] * L[6]) +
+                               L[2] * (L[3] * L[7] - L[4] * L[6]);
+            const double id = det != 0.0 ? 1.0 / det : 0.0;
+            inv[0] = (L[4] * L[8] - L[5] * L[7]) * id;
+            inv[1] = (L[2] * L[7] - L[1] * L[8]) * id;
+            inv[2] = (L[1] * L[5] - L[2] * L[4]) * id;
+            inv[3] = (L[5] * L[6] - L[3] * L[8]) * id;
+            inv[4] = (L[0] * L[8] - L[2] * L[6]) * id;
+            inv[5] = (L[2] * L[3] - L[0] * L[5]) * id;
+            inv[6] = (L[3] * L[7] - L[4] * L[6]) * id;
+            inv[7] = (L[1] * L[6] - L[0] * L[7]) * id;
+            inv[8] = (L[0] * L[4] - L[1] * L[3]) * id;
+            for (int k = 0; k < 3; k++) {
+                lvecs[s * 3 + k] = f4(L[k * 3], L[k * 3 + 1], L[k * 3 + 2], 0);
+                ilvecs[s * 3 + k] = f4(inv[k], inv[3 + k], inv[6 + k], 0);
+            }
+            for (int k = 0; k < I.npbc; k++)
+                pbcshifts[I.sh0 + k] = f4(shifts[k * 3], shifts[k * 3 + 1], shifts[k * 3 + 2], 0);
+        }
     }
     if (!bParams) return;
     const double alpha = grid.alphaMorse;
@@ -229,6 +314,12 @@ void World::pack_system(int s, bool bParams, bool bForces, bool bVel, bool blvec
         neighs[I.a0 + i] = make_int4(m.neighs[i * 4], m.neighs[i * 4 + 1], m.neighs[i * 4 + 2], m.neighs[i * 4 + 3]);
         neighCell[I.a0 + i] = make_int4(-1, -1, -1, -1);
         elem[I.a0 + i] = m.elem.empty() ? 0 : m.elem[i];
+    }
+    if (m.pbc) {  // image of each bonded neighbour at the current geometry (makeNeighCells)
+        std::vector<int> nc((size_t)N * 4);
+        fc::make_neigh_cells(N, m.apos.data(), m.neighs.data(), I.npbc, shifts.data(), nc.data());
+        for (int i = 0; i < N; i++)
+            neighCell[I.a0 + i] = make_int4(nc[i * 4], nc[i * 4 + 1], nc[i * 4 + 2], nc[i * 4 + 3]);
     }
     for (int k = 0; k < M; k++) {
         auto P4 = [&](const std::vector<double>& a) { return f4(a[k * 4], a[k * 4 + 1], a[k * 4 + 2], a[k * 4 + 3]); };
@@ -337,6 +428,8 @@ bool World::upload_all(bool bParams, bool bForces, bool bVel) {
         d_PLQs.upload(PLQs, stream);
         d_neighs.upload(neighs, stream);
         d_bkNeighs.upload(bkNeighs, stream);
+        d_neighCell.upload(neighCell, stream);
+        d_pbcshifts.upload(pbcshifts, stream);
         d_apars.upload(MMpars, stream);
         d_bLs.upload(BLs, stream);
         d_bKs.upload(BKs, stream);
@@ -453,6 +546,8 @@ fcd::BatchDev World::batch_dev() {
     B.sysflags = d_sysflags.p;
     B.gridcache = d_gridcache.p;
     B.gridcache_stride = ntotA;
+    B.pbcshifts = d_pbcshifts.p;
+    B.neighCell = d_neighCell.p;
     B.snap = (snap_on && d_snap.p) ? d_snap.p : nullptr;
     B.snap_every = std::max(snap_every, 1);
     B.nsnap = nsnap;
@@ -485,6 +580,14 @@ fcd::RunParams World::run_params(double Fconv, int nstep_max) {
     P.pc_gstar = pc_gstar;
     P.pc_grigid = pc_grigid;
     P.pc_every = std::max(pc_every, 1);
+    if (!surf_pos.empty()) {
+        P.surf_pos = d_surf_pos.p;
+        P.surf_REQ = d_surf_REQ.p;
+        P.surf_shifts = d_surf_shifts.p;
+        P.nsurf = (int)surf_pos.size();
+        P.nsurf_pbc = (int)surf_shifts.size();
+        P.surf_K = (float)-surf_alpha;
+    }
     if (d_mlpW.p) {
         P.mlp.W1 = d_mlpW.p;
         P.mlp.W2 = d_mlpW.p + 64 * 16;
@@ -523,6 +626,8 @@ void World::plan_classes(bool upload) {
                 c = k;
                 break;
             }
+        // periodic molecules and explicit substrate atoms: the split kernels
+        if (mols[s].pbc || !surf_pos.empty()) c = -1;
         sys_class[s] = c;
         if (c < 0) {
             too_big++;
@@ -534,8 +639,8 @@ void World::plan_classes(bool upload) {
     }
     n_unclassed = too_big;
     if (too_big && verbosity > 0)
-        std::fprintf(stderr, "[libMMFFmulti_lib] %d systems exceed the 256-thread workgroup and are relaxed by the "
-                             "split kernels (plain FIRE)\n", too_big);
+        std::fprintf(stderr, "[libMMFFmulti_lib] %d systems (periodic, over substrate atoms, or above the 256-thread "
+                             "workgroup) are relaxed by the split kernels (plain FIRE)\n", too_big);
     int ncu = 256;
     {
         hipDeviceProp_t prop;
@@ -704,7 +809,8 @@ int World::relax_noalloc(int nstep_max, double Fconv, bool restore_start) {
     if (!ring_ok) return fail("asymmetric exclusions need the split kernels; use relax()");
     if (record_times) return fail("per-system timing allocates; switch it off (set_record_times(0))");
     if (autotune || h_order.empty()) plan_classes(false);
-    if (n_unclassed > 0) return fail("systems above 256 atoms need the split kernels; use relax()");
+    if (n_unclassed > 0)
+        return fail("periodic systems, substrate atoms and systems above 256 atoms need the split kernels; use relax()");
     // scratch layout: counters | order | activation slots | window cache (256 B aligned)
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t off_order = 256;
@@ -925,6 +1031,74 @@ int World::relax_split(int nstep_max, double Fconv, bool restore_start) {
     int mx = 0;
     for (int s = 0; s < nSys; s++) mx = std::max(mx, nsteps[s]);
     return mx;
+}
+
+// Global-optimisation exploring on the split kernels (the reference's
+// run_ocl_opt with bGopt: updateMultiExploring every nPerVFs steps and the
+// TDrive Langevin term in updateAtomsMMFFf4, MolWorld_sp3_multi.h:770-833,
+// relax_multi.cl:1160-1178): every system alternates FIRE relaxation and
+// Langevin exploring on the device (explore_move) for nstep_max steps and
+// records the minima its relaxations reach.  Returns the total number of
+// minima found, -1 on error.
+int World::explore(int nstep_max, double Fconv, double T, double gamma, int n_explore, int n_relax,
+                   unsigned seed, int kmax, bool keep_positions) {
+    if (!dev_ready || nSys == 0) return -1;
+    if (!std::isfinite(T) || !std::isfinite(gamma) || n_explore < 1 || kmax < 1 || nstep_max < 0) {
+        std::fprintf(stderr, "[libMMFFmulti_lib] explore: needs finite T and gamma, nExplore >= 1, kmax >= 1\n");
+        return -1;
+    }
+    // the reference drives only when gamma > 0 (TDrive.y > 0, relax_multi.cl:1155); T < 0 or gamma <= 0
+    // leave the exploring phase undriven (constant-energy dynamics at dt_max)
+    T = std::max(T, 0.0);
+    gamma = std::max(gamma, 0.0);
+    xkmax = kmax;
+    d_xstate.upload(std::vector<int4>(nSys, make_int4(0, 0, 0, 0)), stream);
+    d_minE.alloc((size_t)nSys * kmax);
+    if (keep_positions) d_minpos.alloc((size_t)kmax * ntotV);
+    else d_minpos.free_();
+    fcd::BatchDev B = batch_dev();
+    B.xstate = d_xstate.p;
+    B.minE = d_minE.p;
+    B.minpos = keep_positions ? d_minpos.p : nullptr;
+    fcd::RunParams P = run_params(Fconv, nstep_max);
+    P.pc_mode = 0;
+    P.explore = 1;
+    P.xT = (float)T;
+    P.xgamma = (float)gamma;
+    P.xn_explore = n_explore;
+    P.xn_relax = std::max(n_relax, 0);
+    P.xseed = seed;
+    P.xkmax = kmax;
+    hipMemsetAsync(d_avel.p, 0, sizeof(float4) * ntotV, stream);
+    fc_launch_reset(stream, &B, &P);
+    EventSet events;
+    hipEvent_t e0 = events.make(), e1 = events.make();
+    hipEventRecord(e0, stream);
+    for (int step = 0; step <= nstep_max; step++)
+        if (!hip_ok(fc_launch_split_step(stream, &B, &P, d_node_sys.p, ntotM, maxN, 1, step), "explore step"))
+            return -1;
+    hipEventRecord(e1, stream);
+    // DevArr::download fills the host vector's current size: size them first
+    std::vector<int4> xs((size_t)nSys);
+    minE.assign((size_t)nSys * kmax, 0.0);
+    minpos.assign(keep_positions ? (size_t)kmax * ntotV : 0, make_float4(0, 0, 0, 0));
+    d_xstate.download(xs, stream);
+    d_nsteps.download(nsteps, stream);
+    d_status.download(status, stream);
+    d_minE.download(minE, stream);
+    if (keep_positions) d_minpos.download(minpos, stream);
+    if (!hip_ok(hipStreamSynchronize(stream), "explore")) return -1;
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    last_ms = ms;
+    nmin.assign(nSys, 0);
+    int tot = 0;
+    for (int s = 0; s < nSys; s++) {
+        nmin[s] = xs[s].z;
+        tot += xs[s].z;
+    }
+    minpos_kept = keep_positions;
+    return tot;
 }
 
 bool World::eval_forces() {

@@ -135,6 +135,15 @@ struct World {
     // ---- GridFF
     fc::GridFFHost grid;
     std::vector<float4> grid_f4;
+    // ---- explicit substrate atoms (instead of the GridFF; split kernels)
+    std::vector<float4> surf_pos, surf_REQ, surf_shifts;
+    fc::Surface surf_host;  // the substrate atoms in double precision
+    int surf_nPBC[3] = {0, 0, 0};
+    DevArr<float4> d_surf_pos, d_surf_REQ, d_surf_shifts;
+    double surf_alpha = 1.5;
+    // substrate atoms of s (lattice s.lvec) with images -nPBC..nPBC; replaces
+    // the GridFF of the batch; s.natoms() == 0 removes them
+    bool set_surface_atoms(const fc::Surface& s, const int* nPBC);
 
     // ---- device
     bool dev_ready = false;
@@ -144,7 +153,13 @@ struct World {
     DevArr<float4> d_gridcache;  // per-atom GridFF window cache (persistent kernel)
     DevArr<float4> d_atoms, d_atoms0, d_aforce, d_avel, d_REQs, d_PLQs, d_apars, d_bLs, d_bKs, d_Ksp, d_Kpp, d_fneigh,
         d_fire, d_grid;
-    DevArr<int4> d_neighs, d_bkNeighs;
+    DevArr<int4> d_neighs, d_bkNeighs, d_neighCell;
+    DevArr<float4> d_pbcshifts;
+    int n_periodic = 0;  // periodic molecules in the batch (relaxed by the split kernels)
+    int nPBC_init[3] = {0, 0, 0};  // image range given to init()
+    // lattice (rows a, b, c) and image range of system isys (all if < 0); lvec
+    // null makes it a vacuum molecule again.  Re-packs and uploads the batch.
+    bool set_lattice(int isys, const double* lvec, const int* nPBC);
     DevArr<unsigned> d_excl, d_excl_rot;
     DevArr<fcd::PairCorrDev> d_corr;
     DevArr<fcd::SysInfo> d_info;
@@ -208,6 +223,20 @@ struct World {
     // a system of one batch, relaxed together; the previous batch is restored
     bool scan(int nconf, const double* poss, const double* rots, double* Es, double* aforces, double* aposs,
               bool bRelax, int niter_max, double Fconv, double Flim);
+    // global-optimisation exploring (split kernels): minima per system
+    int explore(int nstep_max, double Fconv, double T, double gamma, int n_explore, int n_relax, unsigned seed,
+                int kmax, bool keep_positions);
+    DevArr<int4> d_xstate;
+    DevArr<double> d_minE;
+    DevArr<float4> d_minpos;
+    std::vector<double> minE;   // [nSys * xkmax]
+    std::vector<float4> minpos;  // [xkmax * ntotV]
+    std::vector<int> nmin;       // minima found per system (may exceed xkmax)
+    int xkmax = 0;
+    bool minpos_kept = false;
+    // exploring parameters of init() (T, gamma, nExplore, nRelax); run() explores when nExplore > 0
+    double go_T = -1, go_gamma = -1;
+    int go_nExplore = 0, go_nRelax = 0;
     bool eval_forces();
     bool eval_precond();
     bool sync();

@@ -25,6 +25,11 @@ ST_RUNNING, ST_CONVERGED, ST_MAXSTEP, ST_NAN = 0, 1, 2, 3
 SYSINFO_FIELDS = ("natoms", "nnode", "a0", "n0", "v0", "mask0", "corr0", "ncorr")
 
 
+# ints per system of the "sys_info" buffer (SysInfo: natoms, nnode, a0, n0, v0,
+# mask0, corr0, ncorr, sh0, npbc)
+SYSINFO_INTS = 10
+
+
 def _p(a, t=c_double_p):
     return None if a is None else a.ctypes.data_as(t)
 
@@ -169,6 +174,94 @@ def make_exclusions(natoms, nnode, neighs, subtract13=True):
     return bits.astype(bool), nc.value
 
 
+def build_xyz(xyz, tables_dir, maxn=4096):
+    """Typed molecule of an .xyz file (the builder of init(xyz_name)) with the
+    type tables of `tables_dir` (ElementTypes.dat, AtomTypes.dat,
+    BondTypes.dat); returns (molecule dict, lvec 3x3 or None)."""
+    import os
+    L = lib()
+    L.build_xyz_arrays.argtypes = [c_char_p] * 4 + [c_int, c_int_p] + [c_void_p] * 11
+    L.build_xyz_arrays.restype = c_int
+    apos = np.zeros((2 * maxn, 3))
+    REQs = np.zeros((maxn, 4))
+    neighs = np.zeros((maxn, 4), np.int32)
+    pars = [np.zeros((maxn, 4)) for _ in range(5)]
+    elem = np.zeros(maxn, np.int32)
+    lv = np.zeros(9)
+    per = c_int(0)
+    nn = c_int(0)
+    t = lambda f: os.path.join(tables_dir, f).encode()
+    n = L.build_xyz_arrays(xyz.encode(), t("ElementTypes.dat"), t("AtomTypes.dat"), t("BondTypes.dat"), maxn,
+                           ctypes.byref(nn), apos.ctypes.data, REQs.ctypes.data, neighs.ctypes.data,
+                           *[p.ctypes.data for p in pars], elem.ctypes.data, lv.ctypes.data, ctypes.addressof(per))
+    if n < 0:
+        raise ValueError("cannot build %s" % xyz)
+    m = nn.value
+    d = dict(natoms=n, nnode=m, apos=np.ascontiguousarray(apos[:n + m]), REQs=np.ascontiguousarray(REQs[:n]),
+             neighs=np.ascontiguousarray(neighs[:n]), elem=np.ascontiguousarray(elem[:n]))
+    for k, p in zip(("apars", "bLs", "bKs", "Ksp", "Kpp"), pars):
+        d[k] = np.ascontiguousarray(p[:m])
+    return d, (lv.reshape(3, 3) if per.value else None)
+
+
+def make_neigh_cells(apos, neighs, lvec, nPBC):
+    """(shifts (npbc,3), neighCell (natoms,4)) of a periodic molecule."""
+    L = lib()
+    L.make_neigh_cells.argtypes = [c_int, c_double_p, c_int_p, c_double_p, c_int_p, c_double_p, c_int_p]
+    L.make_neigh_cells.restype = c_int
+    apos = np.ascontiguousarray(apos, np.float64)
+    neighs = np.ascontiguousarray(neighs, np.int32)
+    n = len(neighs)
+    lv = np.ascontiguousarray(lvec, np.float64).reshape(9)
+    npb = np.ascontiguousarray(nPBC, np.int32)
+    nsh = int(np.prod(2 * npb + 1))
+    sh = np.zeros((nsh, 3))
+    nc = np.zeros((n, 4), np.int32)
+    L.make_neigh_cells(n, _p(apos), _p(neighs, c_int_p), _p(lv), _p(npb, c_int_p), _p(sh), _p(nc, c_int_p))
+    return sh, nc
+
+
+def attach_surface_atoms(surf="synth:NaCl", nPBC=(1, 1, 0)):
+    """The substrate as explicit atoms (no GridFF); surf None removes them."""
+    L = lib()
+    L.attach_surface_atoms.argtypes = [c_char_p, c_int_p]
+    L.attach_surface_atoms.restype = c_int
+    npb = np.ascontiguousarray(nPBC, np.int32)
+    r = L.attach_surface_atoms(None if surf is None else surf.encode(), _p(npb, c_int_p))
+    if r < 0:
+        raise RuntimeError("attach_surface_atoms failed")
+
+
+def surface_atoms(surf=None, maxn=100000):
+    """(apos, REQs, lvec 3x3, nPBC) of the substrate atoms of `surf` read on the
+    host (nPBC None), or of the ones in use (None if none)."""
+    L = lib()
+    L.get_surface_atoms.argtypes = [c_char_p, c_double_p, c_double_p, c_double_p, c_int_p, c_int]
+    L.get_surface_atoms.restype = c_int
+    name = None if surf is None else surf.encode()
+    n = L.get_surface_atoms(name, None, None, None, None, 0)
+    if n < 0:
+        raise RuntimeError("cannot read surface %s" % surf)
+    if n == 0:
+        return None
+    a, q, lv, npb = np.zeros((n, 3)), np.zeros((n, 4)), np.zeros(9), np.zeros(3, np.int32)
+    L.get_surface_atoms(name, _p(a), _p(q), _p(lv), _p(npb, c_int_p), n)
+    return a, q, lv.reshape(3, 3), (None if surf is not None else tuple(int(v) for v in npb))
+
+
+def set_lattice(isys, lvec=None, nPBC=None):
+    """Make system isys (all if < 0) periodic with lattice rows lvec (3x3) and
+    image range nPBC, or a vacuum molecule again (lvec None)."""
+    L = lib()
+    L.set_lattice.argtypes = [c_int, c_double_p, c_int_p]
+    L.set_lattice.restype = c_int
+    lv = None if lvec is None else np.ascontiguousarray(lvec, np.float64).reshape(9)
+    npb = None if nPBC is None else np.ascontiguousarray(nPBC, np.int32)
+    r = L.set_lattice(isys, None if lv is None else _p(lv), None if npb is None else _p(npb, c_int_p))
+    if r < 0:
+        raise RuntimeError("set_lattice failed")
+
+
 def make_nacl_grid(nx=56, ny=56, nz=200, dz=0.1, nthreads=8, Rdamp=1.0):
     """Synthetic NaCl(001) GridFF coefficients (host direct sums + fit); Rdamp =
     Coulomb damping of the node sums (1 A: the benchmark substrate; 0: bare)."""
@@ -255,7 +348,7 @@ class Batch:
         self.sizes = s
         ns, A, M, V = s["nsys"], s["ntotA"], s["ntotM"], s["ntotV"]
         self.nsys = ns
-        self.info = iview("sys_info", ns * 8).reshape(ns, 8)
+        self.info = iview("sys_info", ns * SYSINFO_INTS).reshape(ns, SYSINFO_INTS)
         self.atoms = fview("gpu_atoms", V * 4).reshape(V, 4)
         self.aforces = fview("gpu_aforces", V * 4).reshape(V, 4)
         self.avel = fview("gpu_avel", V * 4).reshape(V, 4)
@@ -322,6 +415,34 @@ def relax_noalloc(nstep_max, Fconv, restore_start=False, download=True):
     if n < 0:
         raise RuntimeError("relax_noalloc failed")
     return n
+
+
+def explore(nstep_max, Fconv, T, gamma, n_explore, n_relax=0, seed=0, kmax=16, positions=True):
+    """Exploring run of the whole batch (split kernels); returns the minima found."""
+    L = lib()
+    L.explore.argtypes = [c_int, c_double, c_double, c_double, c_int, c_int, c_longlong, c_int, c_bool]
+    L.explore.restype = c_int
+    n = L.explore(nstep_max, Fconv, T, gamma, n_explore, n_relax, seed, kmax, positions)
+    if n < 0:
+        raise RuntimeError("explore failed")
+    return n
+
+
+def minima(isys, maxn=16):
+    """(found, energies, positions) of system isys from the last explore()."""
+    L = lib()
+    L.get_minima.argtypes = [c_int, c_double_p, c_double_p, c_int]
+    L.get_minima.restype = c_int
+    s = batch_sizes()
+    info = iview("sys_info", s["nsys"] * SYSINFO_INTS).reshape(s["nsys"], SYSINFO_INTS)
+    V = int(info[isys, 0] + info[isys, 1])
+    E = np.zeros(maxn)
+    P = np.zeros((maxn, V, 3))
+    n = L.get_minima(isys, _p(E), _p(P), maxn)
+    if n < 0:
+        raise RuntimeError("get_minima failed")
+    k = min(n, maxn)
+    return n, E[:k], P[:k]
 
 
 def last_ms():

@@ -89,7 +89,10 @@ void download_sys(int isys, bool bForces, bool bVel);
 void upload(bool bParams, bool bForces, bool bVel, bool blvec);
 /* MMFFmulti_lib.cpp:270-272 */
 void download(bool bForces, bool bVel);
-/* MMFFmulti_lib.cpp:275-280  not supported (periodic molecules): no-op */
+/* MMFFmulti_lib.cpp:275-280 (MolWorld_sp3_multi::change_lvec / add_to_lvec,
+ * MolWorld_sp3_multi.h:1167-1200): bAdd = false sets the lattice of system 0
+ * (the reference's current system), making it periodic with init's nPBC;
+ * bAdd = true adds lvec to the lattice of every periodic system. */
 void change_lvec(double* lvec, bool bAdd, bool bUpdatePi);
 /* MMFFmulti_lib.cpp:290-293  not supported: no-op */
 void optimizeLattice_1d(double* dlvec, int n1, int n2, int initMode, double tol);
@@ -192,6 +195,18 @@ int init_pi_vectors(int natoms, int nnode, const double* apos, const int* neighs
  * current batch, as init(surf_name=...) does at initialisation (GridFF.h:
  * 1787-1866 tryLoad_new).  0 on success, -1 on error. */
 int attach_surface(const char* surf_name, double gridStep);
+/* The substrate as explicit atoms instead of the GridFF (MolWorld_sp3::loadSurf
+ * with bGrid = false; per atom GridFF::evalMorsePBC, GridFF.h:1229-1261: Morse
+ * + Coulomb pairs with every substrate atom at its images -nPBC..nPBC, null =
+ * (1,1,0)), as init(surf_name, GridFF <= 0) selects.  surf_name = null removes
+ * them.  Systems over substrate atoms are relaxed by the split kernels.  0 on
+ * success, -1 on error. */
+int attach_surface_atoms(const char* surf_name, const int* nPBC3);
+/* Substrate atoms: those of surf_name read on the host (no device work), or
+ * with surf_name = null the ones in use.  Positions (n x 3), REQs (n x 4),
+ * lattice rows (9) and image range (3, in-use atoms only), at most maxn atoms.
+ * Returns n (0: none in use), -1 if surf_name cannot be read. */
+int get_surface_atoms(const char* surf_name, double* apos, double* REQs, double* lvec9, int* nPBC3, int maxn);
 /* System i of init_workload (kinds 1,2) generated on the host only. */
 int gen_workload_arrays(int kind, int i, long long seed0, int nmin, int nmax, double sigma, const double* pos0,
                         const double* cell, int* natoms_out, int* nnode_out, double* apos, double* REQs, int* neighs,
@@ -207,11 +222,40 @@ int relax(int nstepMax, double Fconv, int path, bool restore_start, bool bDownlo
  * slots of the in-kernel network, GridFF window cache: 256 B per atom) lives
  * in a static device array of 64 MiB per device, held until the process ends.
  * Returns -1 with a message when the batch needs more scratch (about 250,000
- * atoms) or a path that allocates (systems above 256 atoms, asymmetric
- * exclusions, the chunked learned form, per-system timing).  Calls that
+ * atoms) or a path that allocates (periodic systems, systems above 256 atoms,
+ * asymmetric exclusions, the chunked learned form, per-system timing).  Calls that
  * overlap on one device share its scratch and race.  Results are those of
  * relax(path 0). */
 int relax_noalloc(int nstepMax, double Fconv, bool restore_start, bool bDownload);
+/* Global-optimisation exploring (MolWorld_sp3_multi::updateMultiExploring,
+ * MolWorld_sp3_multi.h:770-833, with the TDrive Langevin term of
+ * updateAtomsMMFFf4, relax_multi.cl:1160-1178; the reference enables it with
+ * init(T, gamma, nExplore, nRelax) and runs it inside run()).  Every system
+ * alternates FIRE relaxation and nExplore steps of Langevin dynamics at
+ * temperature T (K) and damping gamma, for nstepMax steps from its current
+ * state; a relaxation longer than nRelax steps (> 0) is abandoned.  Each
+ * converged relaxation records a minimum (energy; positions if bPositions),
+ * up to kmax per system.  The random force is a counter-based hash of (seed,
+ * system, vec, axis, step).  Returns the minima found over all systems, -1 on
+ * error.  run() takes this path when init() was given nExplore > 0. */
+int explore(int nstepMax, double Fconv, double T, double gamma, int nExplore, int nRelax, long long seed, int kmax,
+            bool bPositions);
+/* Minima of system isys from the last explore(): energies (eV) and, when
+ * recorded, positions (natoms+nnode rows of x, y, z each); at most maxn.
+ * Returns the number found (may exceed kmax), -1 on error. */
+int get_minima(int isys, double* Es, double* poss, int maxn);
+/* Periodic molecules (MMFFsp3_loc with bPBC: the non-bonded sum runs over the
+ * image shifts of evalPBCshifts, MolWorld_sp3.h:756-765, as
+ * NBFF::evalLJQs_ng4_PBC_atom_omp, NBFF.h:575-620; bonded neighbours sit at the
+ * image makeNeighCells picks, MMFFsp3_loc.h:1394-1434).  Sets the lattice rows
+ * lvec9 (a, b, c) and the image range nPBC3 (null: the system's, else init's
+ * nPBC) of system isys, or of all systems if isys < 0; lvec9 = null makes them
+ * vacuum molecules again.  The batch is re-packed and uploaded; periodic
+ * systems are relaxed by the split kernels.  0 on success, -1 on error.  No
+ * counterpart in the reference's C-ABI (whose periodic cell comes from the
+ * molecule's .xyz lvs line and change_lvec) beyond change_lvec, which is
+ * served through it. */
+int set_lattice(int isys, const double* lvec9, const int* nPBC3);
 /* Device time of the last relax (HIP events), ms. */
 double get_last_ms(void);
 /* Queue order of the persistent kernel (longest first) by the estimated
@@ -296,6 +340,21 @@ int gen_molecule_arrays(int natoms, long long seed, double sigma, long long psee
                         double* REQs, int* neighs, double* apars, double* bLs, double* bKs, double* Ksp, double* Kpp,
                         int* elem);
 int make_back_neighs(int natoms, int nnode, const int* neighs, int* bkneighs);
+/* Typed molecule of an .xyz file, built on the host as init(xyz_name) builds
+ * its template (the reference's MolWorld_sp3::init with bMMFF: MMFFBuilder
+ * typing, bonds, toMMFFsp3_loc, MolWorld_sp3.h:309-357, 1143-1440), with the
+ * type tables fElem/fAtom/fBond (ElementTypes.dat, AtomTypes.dat,
+ * BondTypes.dat).  Outputs in the MMFFsp3_loc layout; lvec9 and *periodic
+ * report an lvs line.  Returns natoms, -1 on error, -2 above maxn atoms. */
+int build_xyz_arrays(const char* xyz, const char* fElem, const char* fAtom, const char* fBond, int maxn,
+                     int* nnode_out, double* apos, double* REQs, int* neighs, double* apars, double* bLs, double* bKs,
+                     double* Ksp, double* Kpp, int* elem, double* lvec9, int* periodic);
+/* Periodic molecule: image shifts in evalPBCshifts order (MolWorld_sp3.h:
+ * 756-765) into shifts_out (npbc x 3) and the image of each bonded neighbour
+ * (MMFFsp3_loc::makeNeighCells, MMFFsp3_loc.h:1394-1434) into neighCell_out
+ * (natoms x 4, -1 for empty slots).  Returns npbc. */
+int make_neigh_cells(int natoms, const double* apos, const int* neighs, const double* lvec9, const int* nPBC3,
+                     double* shifts_out, int* neighCell_out);
 int make_exclusions(int natoms, int nnode, const int* neighs, int subtract13, unsigned* mask_out, int* W_out,
                     int* ncorr_out);
 /* Synthetic NaCl(001) slab GridFF (host sums + fit); Rdamp = Coulomb damping

@@ -100,6 +100,131 @@ class Backend:
         n = self._fn("relax")(*args)
         return n, apos, E.value, traj[:ntraj]
 
+    def explore(self, m, fire, Fconv, nmax, T, gamma, n_explore, n_relax=0, seed=0, sysidx=0, kmax=16,
+                sw=(1, 1, 1, 1, 1), grid=None, ntraj=0):
+        """Exploring run of one system (relax / Langevin phases, oracle only):
+        returns (minima found, final positions, minima energies, minima
+        positions, trajectory of the first ntraj steps)."""
+        N, M = m["natoms"], m["nnode"]
+        apos = np.array(m["apos"], dtype=np.float64, copy=True)
+        fire = np.zeros(8) + np.r_[np.asarray(fire, dtype=np.float64), np.zeros(8 - len(fire))]
+        minE = np.zeros(kmax)
+        minpos = np.zeros((kmax, N + M, 3))
+        traj = np.zeros((max(ntraj, 1), N + M, 3))
+        a, sw_keep = self._sys_args(m, sw)
+        f = self.lib.orc_explore
+        f.restype = ctypes.c_int
+        args = a[:2] + [_p(apos)] + a[2:] + self._grid_args(grid) + [
+            _p(fire), ctypes.c_double(Fconv), ctypes.c_int(nmax), ctypes.c_double(T), ctypes.c_double(gamma),
+            ctypes.c_int(n_explore), ctypes.c_int(n_relax), ctypes.c_uint(seed & 0xFFFFFFFF), ctypes.c_int(sysidx),
+            ctypes.c_int(kmax), _p(minE), _p(minpos), _p(traj) if ntraj > 0 else None, ctypes.c_int(ntraj)]
+        n = f(*args)
+        k = max(0, min(n, kmax))
+        return n, apos, minE[:k], minpos[:k], traj[:ntraj]
+
+    def drive_xi(self, seed, sysidx, vec, axis, step):
+        f = self.lib.orc_drive_xi
+        f.restype = ctypes.c_double
+        f.argtypes = [ctypes.c_uint] * 5
+        return f(seed, sysidx, vec, axis, step)
+
+    def set_surface(self, apos=None, REQs=None, lvec=None, nPBC=(1, 1, 0), alpha=1.5):
+        """Explicit substrate atoms (instead of a grid) for the following calls;
+        apos None removes them.  Oracle: images -nPBC..nPBC of lvec; reference
+        build: ref_set_surface (its own image shifts)."""
+        if self.prefix == "ref_":
+            f = self.lib.ref_set_surface
+            f.restype = None
+            if apos is None:
+                f(ctypes.c_int(0), None, None, None, None, ctypes.c_double(alpha))
+                return
+            a = np.ascontiguousarray(apos, np.float64)
+            q = np.ascontiguousarray(REQs, np.float64)
+            lv = np.ascontiguousarray(lvec, np.float64).reshape(9)
+            npb = np.ascontiguousarray(nPBC, np.int32)
+            f(ctypes.c_int(len(a)), _p(a), _p(q), _p(lv), _p(npb, _ip), ctypes.c_double(alpha))
+            return
+        f = self.lib.orc_set_surface
+        f.restype = None
+        if apos is None:
+            f(ctypes.c_int(0), None, None, ctypes.c_int(0), None, ctypes.c_double(alpha))
+            return
+        a = np.ascontiguousarray(apos, np.float64)
+        q = np.ascontiguousarray(REQs, np.float64)
+        L = np.asarray(lvec, np.float64).reshape(3, 3)
+        nx, ny, nz = (int(v) for v in nPBC)
+        sh = np.ascontiguousarray([L[0] * ix + L[1] * iy + L[2] * iz for iz in range(-nz, nz + 1)
+                                   for iy in range(-ny, ny + 1) for ix in range(-nx, nx + 1)], np.float64)
+        f(ctypes.c_int(len(a)), _p(a), _p(q), ctypes.c_int(len(sh)), _p(sh), ctypes.c_double(alpha))
+
+    def set_pbc(self, natoms=0, shifts=None, neighCell=None):
+        """Periodic molecule (image shifts (npbc,3), neighbour images (natoms,4))
+        for the following eval/relax/explore calls on systems of `natoms` atoms
+        (0 clears).  Oracle only; the reference build takes set_pbc_ref."""
+        f = self.lib.orc_set_pbc
+        f.restype = None
+        if natoms <= 0 or shifts is None:
+            f(ctypes.c_int(0), ctypes.c_int(0), None, None)
+            return
+        sh = np.ascontiguousarray(shifts, np.float64)
+        nc = np.ascontiguousarray(neighCell, np.int32)
+        f(ctypes.c_int(natoms), ctypes.c_int(len(sh)), _p(sh), _p(nc, _ip))
+
+    def set_pbc_ref(self, lvec=None, nPBC=None):
+        """Reference build: periodic molecule (lattice rows, image range) for the
+        following calls, its own shifts and neighbour images (None: vacuum)."""
+        f = self.lib.ref_set_pbc
+        f.restype = None
+        if lvec is None:
+            f(None, None)
+            return
+        lv = np.ascontiguousarray(lvec, np.float64).reshape(9)
+        npb = np.ascontiguousarray(nPBC, np.int32)
+        f(_p(lv), _p(npb, _ip))
+
+    def neigh_cells_ref(self, m, lvec, nPBC):
+        """Reference build: (shifts, neighCell) of molecule m."""
+        f = self.lib.ref_neigh_cells
+        f.restype = ctypes.c_int
+        N, M = m["natoms"], m["nnode"]
+        apos = np.ascontiguousarray(m["apos"], np.float64)
+        ng = np.ascontiguousarray(m["neighs"], np.int32)
+        lv = np.ascontiguousarray(lvec, np.float64).reshape(9)
+        npb = np.ascontiguousarray(nPBC, np.int32)
+        nsh = int(np.prod(2 * npb + 1))
+        nc = np.zeros((N, 4), np.int32)
+        sh = np.zeros((nsh, 3))
+        f(ctypes.c_int(N), ctypes.c_int(M), _p(apos), _p(ng, _ip), _p(lv), _p(npb, _ip), _p(nc, _ip), _p(sh))
+        return sh, nc
+
+    def build_xyz_ref(self, res_dir, xyz_noext, nPBC=(1, 1, 0), maxn=4096):
+        """Reference build: the reference's own MolWorld_sp3::init of an .xyz
+        molecule (run in res_dir, where its common_resources/ live); returns
+        (molecule dict with atypes, neighCell, lvec 3x3 or None)."""
+        f = self.lib.ref_build_xyz
+        f.restype = ctypes.c_int
+        apos = np.zeros((2 * maxn, 3))
+        REQs = np.zeros((maxn, 4))
+        neighs = np.zeros((maxn, 4), np.int32)
+        ncell = np.zeros((maxn, 4), np.int32)
+        pars = [np.zeros((maxn, 4)) for _ in range(5)]
+        atypes = np.zeros(maxn, np.int32)
+        lv = np.zeros(9)
+        per = ctypes.c_int(0)
+        nn = ctypes.c_int(0)
+        npb = np.ascontiguousarray(nPBC, np.int32)
+        n = f(res_dir.encode(), xyz_noext.encode(), _p(npb, _ip), ctypes.c_int(maxn), ctypes.byref(nn), _p(apos),
+              _p(REQs), _p(neighs, _ip), _p(ncell, _ip), *[_p(p) for p in pars], _p(atypes, _ip), _p(lv),
+              ctypes.byref(per))
+        if n < 0:
+            raise RuntimeError("reference build of %s failed" % xyz_noext)
+        m = nn.value
+        d = dict(natoms=n, nnode=m, apos=apos[:n + m].copy(), REQs=REQs[:n].copy(), neighs=neighs[:n].copy(),
+                 neighCell=ncell[:n].copy(), atypes=atypes[:n].copy())
+        for k, p in zip(("apars", "bLs", "bKs", "Ksp", "Kpp"), pars):
+            d[k] = p[:m].copy()
+        return d, (lv.reshape(3, 3) if per.value else None)
+
     def set_restraints(self, natoms=0, constr=None, constrK=None, bbox=None):
         """Positional restraints / walls of the GPU update kernel for the
         following eval/relax calls on systems of `natoms` atoms (0 clears).

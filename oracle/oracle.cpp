@@ -28,6 +28,15 @@
 //                     reference call at MolWorld_sp3.h:2426)
 //   orc_relax_batch   one system per thread, dynamic schedule, the structure
 //                     of MolWorld_sp3_multi::run_omp_ocl (:1948-2027)
+//   orc_set_surface   explicit substrate atoms: surf_atom restates
+//                     GridFF::evalMorsePBC (GridFF.h:1229-1261)
+//   orc_set_pbc       periodic molecules: nb_atom_pbc restates
+//                     NBFF::evalLJQs_ng4_PBC_atom_omp (NBFF.h:575-620), bonds
+//                     at their neighCell image (MMFFsp3_loc.h:325-329)
+//   orc_explore       MolWorld_sp3_multi::updateMultiExploring (phase
+//                     machine, MolWorld_sp3_multi.h:770-833) with the TDrive
+//                     Langevin term of updateAtomsMMFFf4 (relax_multi.cl:
+//                     1160-1178); random numbers from a counter-based hash
 //
 // Build: oracle/Makefile -> oracle/liboracle.so
 #include <algorithm>
@@ -68,9 +77,31 @@ struct Sys {
     const double* constr = nullptr;   // natoms x 4 (x, y, z, on if > 0)
     const double* constrK = nullptr;  // natoms x 4 (per-axis stiffness; negative = 0)
     double bbox[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // lower corner, upper corner, wall stiffness
+    // periodic molecule (optional): image shifts and the image of each bonded neighbour
+    int npbc = 0;
+    const double* shifts = nullptr;  // npbc x 3
+    const int* ngC = nullptr;        // natoms x 4
     // work
     std::vector<double> f, fng, fngpi;
 };
+
+// Explicit substrate atoms (instead of the grid) for the following calls
+// (natoms = 0 clears); the arrays are copied.
+struct Surf {
+    int natoms = 0, npbc = 0;
+    std::vector<double> apos, REQs, shifts;
+    double K = -1.5;  // -alphaMorse
+};
+Surf g_surf;
+
+// Periodic molecule for the following orc_eval / orc_relax / orc_explore calls
+// on systems of `natoms` atoms (natoms = 0 clears); the arrays are copied.
+struct Pbc {
+    int natoms = 0, npbc = 0;
+    std::vector<double> shifts;
+    std::vector<int> ngC;
+};
+Pbc g_pbc;
 
 // Restraints for the following orc_eval / orc_relax calls (natoms = 0 clears);
 // the arrays are copied.
@@ -163,6 +194,7 @@ double eval_node(Sys& S, int ia) {
         int ing = ings[i];
         if (ing < 0) break;
         V h = ld(&S.pos[ing * 3]) - pa;
+        if (S.npbc > 0) h += ld(&S.shifts[S.ngC[ia * 4 + i] * 3]);  // bond image, MMFFsp3_loc.h:325-329
         const double l = std::sqrt(dot(h, h));
         h = h * (1.0 / l);
         hs[i] = h;
@@ -224,8 +256,37 @@ double eval_node(Sys& S, int ia) {
     return E;
 }
 
+// evalLJQs_ng4_PBC_atom_omp, NBFF.h:575-620 (adds to fapos[ia], returns
+// 0.5*E): every other atom at every image shift; a bonded neighbour is skipped
+// only at the image of the bond (neighCell).
+double nb_atom_pbc(Sys& S, int ia) {
+    const V pi = ld(&S.pos[ia * 3]);
+    const double* Ri = S.REQ + ia * 4;
+    const int* ng = S.ng + ia * 4;
+    const int* ngC = S.ngC + ia * 4;
+    double E = 0;
+    V fi{0, 0, 0};
+    for (int j = 0; j < S.N; j++) {
+        if (j == ia) continue;
+        const double* Rj = S.REQ + j * 4;
+        const V dp = ld(&S.pos[j * 3]) - pi;
+        for (int ipbc = 0; ipbc < S.npbc; ipbc++) {
+            bool skip = false;
+            for (int k = 0; k < 4; k++) skip |= (j == ng[k]) && (ipbc == ngC[k]);
+            if (skip) continue;
+            V fij;
+            E += pair_ljqh(dp + ld(&S.shifts[ipbc * 3]), Ri[0] + Rj[0], Ri[1] * Rj[1], Ri[2] * Rj[2],
+                           Ri[3] * Rj[3], fij);
+            fi += fij;
+        }
+    }
+    st(&S.f[ia * 3], ld(&S.f[ia * 3]) + fi);
+    return 0.5 * E;
+}
+
 // evalLJQs_ng4_atom_omp, NBFF.h:666-691 (adds to fapos[ia], returns 0.5*E)
 double nb_atom(Sys& S, int ia) {
+    if (S.npbc > 0) return nb_atom_pbc(S, ia);
     const V pi = ld(&S.pos[ia * 3]);
     const double* Ri = S.REQ + ia * 4;
     const int* ng = S.ng + ia * 4;
@@ -331,6 +392,39 @@ void req2plq(const double* REQ, double alpha, double* PLQ) {
     PLQ[3] = e * e * REQ[3];
 }
 
+// Substrate atoms, GridFF::evalMorsePBC (GridFF.h:1229-1261) with
+// addAtomicForceMorseQ (Forces.h:540-561) and combineREQ (Forces.h:263-268);
+// the pair's Morse amplitude E0*e is rounded to float as the reference does
+// (`float Ae`).  Adds the force to fapos[ia], returns the energy.
+double surf_atom(Sys& S, int ia) {
+    const V pi = ld(&S.pos[ia * 3]);
+    const double* Ri = S.REQ + ia * 4;
+    const double K = g_surf.K;
+    double E = 0;
+    V f{0, 0, 0};
+    for (int j = 0; j < g_surf.natoms; j++) {
+        const double* Rj = &g_surf.REQs[j * 4];
+        const double r0 = Rj[0] + Ri[0], E0 = Rj[1] * Ri[1], qq = Rj[2] * Ri[2];
+        const V dp0 = pi - ld(&g_surf.apos[j * 3]);
+        for (int ipbc = 0; ipbc < g_surf.npbc; ipbc++) {
+            const V dp = dp0 + ld(&g_surf.shifts[ipbc * 3]);
+            const double r2 = dot(dp, dp);
+            const double ir2 = 1 / r2;
+            const double r = std::sqrt(r2 + 1e-32);
+            const double ir = std::sqrt(ir2);
+            const double e = std::exp(K * (r - r0));
+            const float Ae = (float)(E0 * e);
+            const double fM = Ae * 2 * K * (e - 1);
+            const double EM = Ae * (e - 2);
+            const double Eel = COULOMB_K * qq * ir;
+            f -= dp * (fM / r - Eel * ir2);
+            E += EM + Eel;
+        }
+    }
+    st(&S.f[ia * 3], ld(&S.f[ia * 3]) + f);
+    return E;
+}
+
 // eval_no_omp (MolWorld_sp3.h:2178-2245) + assemble_atom (MMFFsp3_loc.h:998-1021)
 double eval_sys(Sys& S) {
     const int N = S.N, M = S.M;
@@ -349,6 +443,8 @@ double eval_sys(Sys& S) {
             V f = ld(&S.f[ia * 3]);
             E += grid_sample(S, ld(&S.pos[ia * 3]), plq, f);
             st(&S.f[ia * 3], f);
+        } else if (g_surf.natoms > 0) {  // bSurfAtoms without bGridFF (MolWorld_sp3.h:2225-2229)
+            E += surf_atom(S, ia);
         }
     }
     // back-neighbours as in makeBackNeighs (first-empty fill over nodes)
@@ -418,6 +514,11 @@ void setup(Sys& S, int natoms, int nnode, const double* apos, const double* REQs
     S.doPiSigma = sw[2];
     S.doAngles = sw[3];
     S.bNonBonded = sw[4];
+    if (g_pbc.natoms == natoms && g_pbc.npbc > 0) {
+        S.npbc = g_pbc.npbc;
+        S.shifts = g_pbc.shifts.data();
+        S.ngC = g_pbc.ngC.data();
+    }
     if (g_restraints.natoms == natoms) {
         S.constr = g_restraints.constr.data();
         S.constrK = g_restraints.constrK.data();
@@ -534,9 +635,157 @@ int relax_sys(Sys& S, const double* fire, double Fconv, int nmax, int conv_mode,
     return itr;
 }
 
+// Random number of the exploring drive: the counter-based hash of (seed,
+// system, vec, axis, step) the device draws (murmur3 32-bit finaliser after
+// each field), as a uniform on [-sqrt(3), sqrt(3)).
+uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+double drive_xi(uint32_t seed, uint32_t sys, uint32_t vec, uint32_t axis, uint32_t step) {
+    uint32_t h = fmix32(seed + 0x9E3779B9u);
+    h = fmix32(h ^ sys);
+    h = fmix32(h ^ (vec * 3u + axis));
+    h = fmix32(h ^ step) >> 8;
+    return ((double)h * (2.0 / 16777216.0) - 1.0) * std::sqrt(3.0);
+}
+
+// Exploring (global optimisation) of one system: the phase machine of
+// MolWorld_sp3_multi::updateMultiExploring (MolWorld_sp3_multi.h:770-833) per
+// step, with the Langevin drive of updateAtomsMMFFf4 (relax_multi.cl:
+// 1160-1178: f += -gamma*v + xi*sqrt(2*kB*T*gamma/dt) on atoms) and the
+// exploring MD parameters of evalVF_new (:755-760: dt = dt_max, velocity kept,
+// no FIRE mixing).  Relaxing phases run FIRE as relax_sys; a converged one
+// records a minimum (energy, positions) and starts an exploring phase of
+// nExplore steps (so does a relaxation longer than nRelax > 0 steps, without a
+// record); after it FIRE restarts from dt_max, damp_max.  Runs steps 0..nmax.
+int explore_sys(Sys& S, const double* fire, double Fconv, int nmax, double T, double gamma, int nExplore, int nRelax,
+                uint32_t seed, uint32_t sysidx, int kmax, double* minE, double* minpos, double* traj, int ntraj) {
+    const double kB = 8.617333262145e-5;  // eV/K, cpp/common/constants.h:12
+    const double Flimit = fire[7];
+    const int N = S.N, V_ = S.N + S.M;
+    Fire F{};
+    F.dt_max = fire[0];
+    F.dt_min = fire[1];
+    F.damp_max = fire[2];
+    F.finc = fire[3];
+    F.fdec = fire[4];
+    F.falpha = fire[5];
+    F.minLastNeg = (int)fire[6];
+    auto fire_reset = [&] {
+        F.dt = F.dt_max;
+        F.damping = F.damp_max;
+        F.lastNeg = 0;
+    };
+    fire_reset();
+    std::vector<double> vel((size_t)V_ * 3, 0.0);
+    const double F2conv = Fconv * Fconv;
+    int phase = 0, pstep = 0, nfound = 0;
+    for (int step = 0; step <= nmax; step++) {
+        const double E = eval_sys(S);
+        double fm2 = 0, vf = 0, vv = 0, ff = 0;
+        for (int i = 0; i < V_; i++) fm2 = std::max(fm2, dot(ld(&S.f[i * 3]), ld(&S.f[i * 3])));
+        if (Flimit > 0)
+            for (int i = 0; i < V_; i++) {
+                V f = ld(&S.f[i * 3]);
+                double f2 = dot(f, f);
+                if (f2 > Flimit * Flimit) st(&S.f[i * 3], f * (Flimit / std::sqrt(f2)));
+            }
+        for (int i = 0; i < V_ * 3; i++) {
+            vf += vel[i] * S.f[i];
+            vv += vel[i] * vel[i];
+            ff += S.f[i] * S.f[i];
+        }
+        if (!std::isfinite(ff)) return -1;
+        bool reset = false;
+        if (phase == 0) {
+            const bool conv = fm2 < F2conv;
+            if (conv || (nRelax > 0 && pstep >= nRelax)) {
+                if (conv) {
+                    if (nfound < kmax) {
+                        if (minE) minE[nfound] = E;
+                        if (minpos) std::memcpy(minpos + (size_t)nfound * V_ * 3, S.pos.data(), sizeof(double) * V_ * 3);
+                    }
+                    nfound++;
+                }
+                phase = 1;
+                pstep = 0;
+            }
+        } else if (pstep >= nExplore) {
+            phase = 0;
+            pstep = 0;
+            reset = true;
+        }
+        if (step >= nmax) break;
+        const bool drive = phase == 1;
+        double dt, cv, cfe;
+        if (drive) {
+            dt = (double)(float)F.dt_max;
+            cv = 1.0;
+            cfe = 0.0;
+        } else {
+            if (reset) fire_reset();
+            fire_update(F, vf, vv, ff);
+            dt = (double)(float)F.dt;
+            cv = F.cv;
+            cfe = F.renorm * F.cf;
+        }
+        const double amp = drive ? std::sqrt(2.0 * kB * std::max(T, 0.0) * std::max(gamma, 0.0) / dt) : 0.0;
+        for (int i = 0; i < V_; i++) {
+            V f = ld(&S.f[i * 3]);
+            V v = ld(&vel[i * 3]);
+            V p = ld(&S.pos[i * 3]);
+            const bool bPi = i >= N;
+            if (bPi) {
+                f -= p * dot(p, f);
+            } else if (drive) {
+                f -= v * std::max(gamma, 0.0);
+                f += V{amp * drive_xi(seed, sysidx, (uint32_t)i, 0u, (uint32_t)step),
+                       amp * drive_xi(seed, sysidx, (uint32_t)i, 1u, (uint32_t)step),
+                       amp * drive_xi(seed, sysidx, (uint32_t)i, 2u, (uint32_t)step)};
+            }
+            v = v * cv;
+            v += f * (dt + cfe);
+            if (bPi) v -= p * dot(p, v);
+            p += v * dt;
+            if (bPi) p = p * (1.0 / std::sqrt(dot(p, p)));
+            st(&vel[i * 3], v);
+            st(&S.pos[i * 3], p);
+        }
+        pstep++;
+        if (traj && step < ntraj) std::memcpy(traj + (size_t)step * V_ * 3, S.pos.data(), sizeof(double) * V_ * 3);
+    }
+    return nfound;
+}
+
 }  // namespace
 
 extern "C" {
+
+// Exploring run of one system (explore_sys); apos updated in place.  Returns
+// the number of minima found, -1 if the system blew up.
+int orc_explore(int natoms, int nnode, double* apos, const double* REQs, const int* neighs, const double* apars,
+                const double* bLs, const double* bKs, const double* Ksp, const double* Kpp, const int* sw, int gnx,
+                int gny, int gnz, const double* gpos0, const double* gstep, const double* gcoefs, double alpha,
+                const double* fire, double Fconv, int nmax, double T, double gamma, int nExplore, int nRelax,
+                unsigned seed, int sysidx, int kmax, double* minE, double* minpos, double* traj, int ntraj) {
+    Sys S;
+    setup(S, natoms, nnode, apos, REQs, neighs, apars, bLs, bKs, Ksp, Kpp, sw, gnx, gny, gnz, gpos0, gstep, gcoefs,
+          alpha);
+    int n = explore_sys(S, fire, Fconv, nmax, T, gamma, nExplore, nRelax, seed, (uint32_t)sysidx, kmax, minE, minpos,
+                        traj, ntraj);
+    std::memcpy(apos, S.pos.data(), sizeof(double) * S.pos.size());
+    return n;
+}
+
+// The exploring drive's random numbers, for the tests of its statistics.
+double orc_drive_xi(unsigned seed, unsigned sys, unsigned vec, unsigned axis, unsigned step) {
+    return drive_xi(seed, sys, vec, axis, step);
+}
 
 int orc_make_back_neighs(int natoms, int nnode, const int* neighs, int* bk, int* neighs_out) {
     for (int i = 0; i < natoms * 4; i++) bk[i] = -1;
@@ -559,6 +808,25 @@ int orc_make_back_neighs(int natoms, int nnode, const int* neighs, int* bk, int*
         }
     }
     return 0;
+}
+
+void orc_set_surface(int natoms, const double* apos, const double* REQs, int npbc, const double* shifts,
+                     double alpha) {
+    g_surf.natoms = natoms;
+    if (natoms <= 0) return;
+    g_surf.npbc = npbc;
+    g_surf.apos.assign(apos, apos + natoms * 3);
+    g_surf.REQs.assign(REQs, REQs + natoms * 4);
+    g_surf.shifts.assign(shifts, shifts + npbc * 3);
+    g_surf.K = -alpha;
+}
+
+void orc_set_pbc(int natoms, int npbc, const double* shifts, const int* neighCell) {
+    g_pbc.natoms = natoms;
+    g_pbc.npbc = npbc;
+    if (natoms <= 0 || npbc <= 0) return;
+    g_pbc.shifts.assign(shifts, shifts + npbc * 3);
+    g_pbc.ngC.assign(neighCell, neighCell + natoms * 4);
 }
 
 void orc_set_restraints(int natoms, const double* constr, const double* constrK, const double* bbox9) {

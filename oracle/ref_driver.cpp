@@ -15,12 +15,35 @@
 #include "MMFFsp3_loc.h"
 #include "GridFF.h"
 #include "DynamicOpt.h"
+#include "MolWorld_sp3.h"
+
+#include <unistd.h>
 
 #include <cmath>
 #include <cstring>
 #include <vector>
 
 namespace {
+
+// periodic molecule for the following ref_eval / ref_relax calls (ref_set_pbc)
+struct RefPbc {
+    bool on = false;
+    double lvec[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int nPBC[3] = {0, 0, 0};
+};
+RefPbc g_ref_pbc;
+
+// explicit substrate atoms for the following calls (ref_set_surface)
+struct RefSurf {
+    int natoms = 0;
+    std::vector<Vec3d> apos;
+    std::vector<Quat4d> REQs;
+    std::vector<int> atypes;
+    double lvec[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int nPBC[3] = {0, 0, 0};
+    double alpha = 1.5;
+};
+RefSurf g_ref_surf;
 
 struct RefSys {
     MMFFsp3_loc ffl;
@@ -56,6 +79,16 @@ struct RefSys {
         ffl.bNonBonded = bNonBonded;
         ffl.setNonBondStrategy(+1);  // bNonBondNeighs
         ffl.makeBackNeighs();
+        if (g_ref_pbc.on) {  // MolWorld_sp3::makeMMFFs PBC set-up (MolWorld_sp3.h:1416-1432)
+            // npbc = makePBCshifts(nPBC, lvec); ffl.bindShifts(npbc, shifts);
+            // ffl.makeNeighCells(npbc, shifts) (NBFF::makePBCshifts returns the
+            // count without storing it, so it is bound as MolWorld_sp3 does)
+            ffl.setLvec(*(const Mat3d*)g_ref_pbc.lvec);
+            const int npbc = ffl.makePBCshifts(Vec3i{g_ref_pbc.nPBC[0], g_ref_pbc.nPBC[1], g_ref_pbc.nPBC[2]}, true);
+            ffl.bindShifts(npbc, ffl.shifts);
+            ffl.makeNeighCells(npbc, ffl.shifts);
+            ffl.bPBC = true;
+        }
     }
     void setup_grid(int nx, int ny, int nz, const double* pos0, const double* step, double* coefs, double alpha) {
         bGrid = true;
@@ -70,6 +103,25 @@ struct RefSys {
         PLQd.resize(ffl.natoms);
         for (int i = 0; i < ffl.natoms; i++) PLQd[i] = REQ2PLQ_d(ffl.REQs[i], alpha);
     }
+    // substrate atoms: MolWorld_sp3::loadSurf without a grid (MolWorld_sp3.h:
+    // 1072-1075): gridFF.bindSystem, gridFF.lvec, makePBCshifts(nPBC, lvec)
+    GridFF gsurf;
+    bool bSurf = false;
+    void setup_surf() {
+        if (g_ref_surf.natoms <= 0) return;
+        bSurf = true;
+        gsurf.bindSystem(g_ref_surf.natoms, g_ref_surf.atypes.data(), g_ref_surf.apos.data(), g_ref_surf.REQs.data());
+        gsurf.lvec = *(const Mat3d*)g_ref_surf.lvec;
+        gsurf.makePBCshifts(Vec3i{g_ref_surf.nPBC[0], g_ref_surf.nPBC[1], g_ref_surf.nPBC[2]}, gsurf.lvec);
+        gsurf.alphaMorse = g_ref_surf.alpha;
+        gsurf.shift0 = Vec3dZero;
+    }
+    ~RefSys() {  // the substrate arrays belong to g_ref_surf
+        gsurf.natoms = 0;
+        gsurf.apos = nullptr;
+        gsurf.REQs = nullptr;
+        gsurf.atypes = nullptr;
+    }
     // MolWorld_sp3::eval_no_omp (MolWorld_sp3.h:2178-2245) recipe
     double eval() {
         double E = 0;
@@ -78,8 +130,9 @@ struct RefSys {
             ffl.fapos[ia] = Vec3dZero;
             if (ia < ffl.nnode) ffl.fapos[ia + ffl.natoms] = Vec3dZero;
             if (ia < ffl.nnode) E += ffl.eval_atom(ia);
-            if (bNonBonded) E += ffl.evalLJQs_ng4_atom_omp(ia);
+            if (bNonBonded) E += ffl.bPBC ? ffl.evalLJQs_ng4_PBC_atom_omp(ia) : ffl.evalLJQs_ng4_atom_omp(ia);
             if (bGrid) E += gff.addAtom(ffl.apos[ia], PLQd[ia], ffl.fapos[ia]);
+            else if (bSurf) E += gsurf.evalMorsePBC_sym(ffl.apos[ia], ffl.REQs[ia], ffl.fapos[ia]);
         }
         for (int ia = 0; ia < ffl.natoms; ia++) ffl.assemble_atom(ia);
         return E;
@@ -111,6 +164,108 @@ void copy_out(const RefSys& R, double* fapos, double* fneigh, double* fneighpi) 
 
 extern "C" {
 
+// Explicit substrate atoms (positions, REQs, lattice rows, image range, Morse
+// alpha) for the following calls; natoms = 0 removes them.
+void ref_set_surface(int natoms, const double* apos, const double* REQs, const double* lvec9, const int* nPBC3,
+                     double alpha) {
+    g_ref_surf.natoms = natoms;
+    if (natoms <= 0) return;
+    g_ref_surf.apos.resize(natoms);
+    g_ref_surf.REQs.resize(natoms);
+    g_ref_surf.atypes.assign(natoms, 0);
+    for (int i = 0; i < natoms; i++) {
+        g_ref_surf.apos[i] = Vec3d{apos[i * 3], apos[i * 3 + 1], apos[i * 3 + 2]};
+        g_ref_surf.REQs[i] = Quat4d{REQs[i * 4], REQs[i * 4 + 1], REQs[i * 4 + 2], REQs[i * 4 + 3]};
+    }
+    for (int k = 0; k < 9; k++) g_ref_surf.lvec[k] = lvec9[k];
+    for (int k = 0; k < 3; k++) g_ref_surf.nPBC[k] = nPBC3[k];
+    g_ref_surf.alpha = alpha;
+}
+
+// Periodic molecule (lattice rows, image range) for the following calls;
+// lvec9 = null switches back to vacuum molecules.
+void ref_set_pbc(const double* lvec9, const int* nPBC3) {
+    g_ref_pbc.on = lvec9 != nullptr;
+    if (!lvec9) return;
+    for (int k = 0; k < 9; k++) g_ref_pbc.lvec[k] = lvec9[k];
+    for (int k = 0; k < 3; k++) g_ref_pbc.nPBC[k] = nPBC3[k];
+}
+
+// The reference's own MMFF build of an .xyz molecule: MolWorld_sp3::init with
+// xyz_name and bMMFF (MolWorld_sp3.h:309-357), run with the working directory
+// set to res_dir (where its common_resources/ type tables live).  Copies out
+// the MMFFsp3_loc arrays: apos (nvecs x 3), REQs (natoms x 4), neighs and
+// neighCell (natoms x 4), apars/bLs/bKs/Ksp/Kpp (nnode x 4), atom types
+// (natoms), lattice (9) and the periodic flag.  Returns natoms (nnode in
+// *nnode_out), -1 if it does not fit maxn.  The world is left allocated: its
+// destructor is not run.
+int ref_build_xyz(const char* res_dir, const char* xyz_noext, const int* nPBC3, int maxn, int* nnode_out,
+                  double* apos, double* REQs, int* neighs, int* neighCell, double* apars, double* bLs, double* bKs,
+                  double* Ksp, double* Kpp, int* atypes, double* lvec9, int* periodic) {
+    char cwd[4096];
+    if (!getcwd(cwd, sizeof(cwd))) return -1;
+    if (chdir(res_dir) != 0) return -1;
+    MolWorld_sp3* W = new MolWorld_sp3();
+    W->xyz_name = xyz_noext;
+    W->bMMFF = true;
+    W->bEpairs = false;
+    W->nPBC = Vec3i{nPBC3[0], nPBC3[1], nPBC3[2]};
+    W->init();
+    if (chdir(cwd) != 0) return -1;
+    const MMFFsp3_loc& f = W->ffl;
+    const int N = f.natoms, M = f.nnode;
+    if (N > maxn) return -1;
+    *nnode_out = M;
+    for (int i = 0; i < f.nvecs; i++) {
+        apos[i * 3] = f.apos[i].x;
+        apos[i * 3 + 1] = f.apos[i].y;
+        apos[i * 3 + 2] = f.apos[i].z;
+    }
+    for (int i = 0; i < N; i++) {
+        for (int k = 0; k < 4; k++) {
+            REQs[i * 4 + k] = f.REQs[i].array[k];
+            neighs[i * 4 + k] = f.neighs[i].array[k];
+            neighCell[i * 4 + k] = f.neighCell[i].array[k];
+        }
+        atypes[i] = f.atypes[i];
+    }
+    for (int i = 0; i < M; i++)
+        for (int k = 0; k < 4; k++) {
+            apars[i * 4 + k] = f.apars[i].array[k];
+            bLs[i * 4 + k] = f.bLs[i].array[k];
+            bKs[i * 4 + k] = f.bKs[i].array[k];
+            Ksp[i * 4 + k] = f.Ksp[i].array[k];
+            Kpp[i * 4 + k] = f.Kpp[i].array[k];
+        }
+    const Mat3d& L = W->builder.lvec;
+    const double lv[9] = {L.a.x, L.a.y, L.a.z, L.b.x, L.b.y, L.b.z, L.c.x, L.c.y, L.c.z};
+    for (int k = 0; k < 9; k++) lvec9[k] = lv[k];
+    *periodic = W->bPBC ? 1 : 0;
+    return N;
+}
+
+// Image shifts (NBFF::makePBCshifts) and neighbour images
+// (MMFFsp3_loc::makeNeighCells(npbc, shifts)) of a periodic molecule.
+int ref_neigh_cells(int natoms, int nnode, const double* apos, const int* neighs, const double* lvec9,
+                    const int* nPBC3, int* neighCell_out, double* shifts_out) {
+    MMFFsp3_loc ffl;
+    ffl.realloc(nnode, natoms - nnode);
+    for (int i = 0; i < ffl.nvecs; i++) ffl.apos[i] = Vec3d{apos[i * 3], apos[i * 3 + 1], apos[i * 3 + 2]};
+    for (int i = 0; i < natoms; i++)
+        ffl.neighs[i] = Quat4i{neighs[i * 4], neighs[i * 4 + 1], neighs[i * 4 + 2], neighs[i * 4 + 3]};
+    ffl.setLvec(*(const Mat3d*)lvec9);
+    const int npbc = ffl.makePBCshifts(Vec3i{nPBC3[0], nPBC3[1], nPBC3[2]}, true);
+    ffl.makeNeighCells(npbc, ffl.shifts);
+    for (int i = 0; i < natoms; i++)
+        for (int k = 0; k < 4; k++) neighCell_out[i * 4 + k] = ffl.neighCell[i].array[k];
+    for (int i = 0; i < npbc; i++) {
+        shifts_out[i * 3] = ffl.shifts[i].x;
+        shifts_out[i * 3 + 1] = ffl.shifts[i].y;
+        shifts_out[i * 3 + 2] = ffl.shifts[i].z;
+    }
+    return npbc;
+}
+
 int ref_make_back_neighs(int natoms, int nnode, const int* neighs, int* bk_out, int* neighs_out) {
     MMFFsp3_loc ffl;
     ffl.realloc(nnode, natoms - nnode);
@@ -133,6 +288,7 @@ double ref_eval(int natoms, int nnode, const double* apos, const double* REQs, c
                 double* fapos, double* fneigh, double* fneighpi) {
     RefSys R;
     R.setup(natoms, nnode, apos, REQs, neighs, apars, bLs, bKs, Ksp, Kpp, sw);
+    R.setup_surf();
     if (gcoefs) R.setup_grid(gnx, gny, gnz, gpos0, gstep, gcoefs, alpha);
     double E = R.eval();
     copy_out(R, fapos, fneigh, fneighpi);
@@ -173,6 +329,7 @@ int ref_relax(int natoms, int nnode, double* apos, const double* REQs, const int
               const double* fire, double Fconv, int nmax, int conv_mode, double* E_out, double* traj, int ntraj) {
     RefSys R;
     R.setup(natoms, nnode, apos, REQs, neighs, apars, bLs, bKs, Ksp, Kpp, sw);
+    R.setup_surf();
     if (gcoefs) R.setup_grid(gnx, gny, gnz, gpos0, gstep, gcoefs, alpha);
     MMFFsp3_loc& ffl = R.ffl;
     DynamicOpt& opt = R.opt;
