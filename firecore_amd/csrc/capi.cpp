@@ -1102,6 +1102,25 @@ int set_proposal_every(int every) {
     world().pc_every = every;
     return 0;
 }
+// FIRE fallback of the learned form: a system whose max|F| at `windows`
+// proposal points in a row is no lower than its best one so far continues
+// with plain FIRE (zero gains, no further network evaluations).  0 = never.
+int set_proposal_stall(int windows) {
+    if (windows < 0) return -1;
+    world().pc_stall = windows;
+    return 0;
+}
+// Per system: 1 when the learned form fell back to plain FIRE in the last
+// relaxation, else 0 (learned form only).  0 on success.
+int get_proposal_fallback(int* out) {
+    World& W = world();
+    if (!W.dev_ready || !out) return -1;
+    std::vector<float4> st(W.nSys);
+    W.d_stall.download(st, W.stream);
+    if (!fcw::hip_ok(hipStreamSynchronize(W.stream), "fallback flags")) return -1;
+    for (int s = 0; s < W.nSys; s++) out[s] = st[s].z != 0.f ? 1 : 0;
+    return 0;
+}
 // Record trajectory snapshots during the next relax() calls: positions of
 // every vec at steps 0, every, 2*every, ... (n snapshots at most).  n = 0
 // stops recording.  0 on success.

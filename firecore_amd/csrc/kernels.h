@@ -21,9 +21,14 @@ struct SysInfo {
     int sh0, npbc;     // periodic molecule: its image shifts pbcshifts[sh0 .. sh0+npbc); npbc = 0 in vacuum
 };
 
+// Weighted pair correction of the non-bonded sum for atom i against atom j.
+// path < 0: at the nearest image (vacuum rings); path = c*16 + ki*4 + kj: a
+// 1-3 pair of a periodic molecule through node c, neighbour slots ki (atom i)
+// and kj (atom j), at the image neighCell[c][kj] - neighCell[c][ki].
 struct PairCorrDev {
     int i, j;
-    float w, pad;
+    float w;
+    int path;
 };
 
 // Status codes written per system.
@@ -59,6 +64,7 @@ struct BatchDev {
     double* Esys;
     float* Fmax;
     float4* fire;          // per-system FIRE state (dt, damping, lastNeg, -) for the split path
+    float4* stall;         // per-system learned-form fallback state (best max|F|^2, count, fallen, -)
     // preconditioner / proposal network inputs
     const int* comp;       // [sum N] connected-component index of each atom (< PC_MAXC)
     const int* ncomp;      // [nsys] number of components
@@ -126,6 +132,8 @@ struct RunParams {
     int pc_mode;
     float pc_gstar, pc_grigid;  // constant gains (mode 1) or gain ceilings (mode 2)
     int pc_every;               // mode 2: steps between network evaluations
+    int pc_stall;               // mode 2: proposal points without a new best max|F| before a system
+                                // falls back to plain FIRE (0 = never)
     MlpDev mlp;
     // chunked execution: > 0 = every system runs at most `chunk` steps per
     // launch, resuming from (and suspending to) the state in B (positions,

@@ -118,6 +118,9 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
     // preconditioner: per-wave partial sums and per-component totals (PC_NSUM values each)
     __shared__ float s_cred[PC ? NW : 1][PC ? PC_MAXC : 1][PC_NSUM];
     __shared__ float s_csum[PC ? PC_MAXC : 1][PC_NSUM];
+    __shared__ float s_smax[NW];
+    __shared__ float s_st_best;       // FIRE fallback of the learned form (below)
+    __shared__ int s_st_cnt, s_st_plain;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -186,10 +189,46 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                 if (v < V) vel[r] = xyz(B.avel[I.v0 + v]);
             }
         }
-        if (learned && !inkernel) {  // chunked: gains from the network kernel between chunks
-            for (int k = tid; k < M; k += BLOCK) s_gain[k] = B.gnode[I.n0 + k];
-            if (tid < PC_MAXC) s_gain[M + tid] = B.gcomp[s * PC_MAXC + tid];
+        // FIRE fallback of the learned form: the best block max|F|^2 seen at a
+        // proposal point after step 0, the proposal points since it was last
+        // improved, and whether the system has fallen back to plain FIRE (zero
+        // gains, no more network evaluations) after P.pc_stall such points.
+        // Chunks of the chunked form carry it in B.stall.
+        // The state lives in LDS, written by thread 0 between barriers only.
+        if (learned) {
+            const bool resume = P.chunk > 0 && step > 0;
+            const float4 t = resume ? B.stall[s] : make_float4(INFINITY, 0.f, 0.f, 0.f);
+            if (tid == 0) {
+                s_st_best = t.x;
+                s_st_cnt = (int)t.y;
+                s_st_plain = t.z != 0.f ? 1 : 0;
+            }
+            if (!inkernel) {  // chunked: gains from the network kernel between chunks
+                const bool plain = t.z != 0.f;
+                for (int k = tid; k < M; k += BLOCK) s_gain[k] = plain ? 0.f : B.gnode[I.n0 + k];
+                if (tid < PC_MAXC) s_gain[M + tid] = plain ? 0.f : B.gcomp[s * PC_MAXC + tid];
+            }
         }
+        // called by every thread of the block with its own max|F|^2 at a proposal point
+        auto stall_check = [&](float fm2t) {
+            fm2t = wave_max(fm2t);
+            if (lane == 0) s_smax[wid] = fm2t;
+            __syncthreads();
+            if (tid == 0) {
+                float m = s_smax[0];
+                for (int w = 1; w < NW; w++) m = fmaxf(m, s_smax[w]);
+                if (m < s_st_best) {
+                    s_st_best = m;
+                    s_st_cnt = 0;
+                } else if (++s_st_cnt >= P.pc_stall) {
+                    s_st_plain = 1;
+                }
+            }
+            __syncthreads();
+            if (s_st_plain)
+                for (int k = tid; k < M + PC_MAXC; k += BLOCK) s_gain[k] = 0.f;
+            __syncthreads();
+        };
         const int step_end = (P.chunk > 0) ? step + P.chunk : 0x7fffffff;
         bool suspended = false;
         __syncthreads();
@@ -413,6 +452,11 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                         }
                     }
                 }
+                // a proposal point after step 0 (in-kernel: before the network runs;
+                // chunked: at the start of a chunk, after the network kernel)
+                if (learned && P.pc_stall > 0 && step > 0 && step % P.pc_every == 0 && (inkernel ? !fresh : fresh) &&
+                    !s_st_plain)
+                    stall_check(fm2);
                 if (!fresh && step >= step_end) {
                     // chunk done: suspend with the exact forces of the current
                     // geometry (written back below), so that the network between
@@ -420,7 +464,7 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
                     suspended = true;
                     break;
                 }
-                if (PC == 2 && inkernel && !fresh && step % P.pc_every == 0) {  // uniform across the block
+                if (PC == 2 && inkernel && !fresh && step % P.pc_every == 0 && !s_st_plain) {  // uniform across the block
                     if (hasAtom) s_fnb[tid] = make_float4(fv[0].x, fv[0].y, fv[0].z, 0.f);
 #pragma unroll
                     for (int r = 0; r < 2; r++) {
@@ -579,6 +623,7 @@ __global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, 
             B.fire[s] = make_float4(fs.dt, fs.damping, (float)fs.lastNeg, 0.f);
             B.nsteps[s] = step;
         }
+        if (learned && tid == 0) B.stall[s] = make_float4(s_st_best, (float)s_st_cnt, s_st_plain ? 1.f : 0.f, 0.f);
         // ---- write back the final state of this system
 #pragma unroll
         for (int r = 0; r < 2; r++) {

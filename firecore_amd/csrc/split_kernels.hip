@@ -13,37 +13,28 @@ namespace fcd {
 // ---------------------------------------------------------------------------
 // Non-bonded force on atom i of a periodic molecule, NBFF::
 // evalLJQs_ng4_PBC_atom_omp (NBFF.h:575-620): every other atom j at every image
-// shift.  A pair the exclusion matrix marks (bonded or 1-3, net weight 0) is
-// left out at its nearest image only: the reference skips a bonded pair at the
-// image of the bond (neighCell) and subtracts a 1-3 pair at the image of its
-// angle's two bonds, which are the nearest images for molecules whose bonded
-// pairs are shorter than half the cell.  Weighted corrections (rings) act at
-// that image too.  Adds the force to f and the unhalved energy to e.
+// shift, a bonded neighbour skipped only at the image of its bond (neighCell).
+// The 1-3 pairs are then subtracted once per angle path at the image reached
+// through the two bonds, as MMFFsp3_loc::eval_atom does in its angle loop
+// (MMFFsp3_loc.h:525-536): the correction list of a periodic system holds one
+// entry per path and end atom, dp = pj - pi + S[neighCell[c][kj]] -
+// S[neighCell[c][ki]] (which may be an image outside the summed range, or the
+// image of a 1-2 bond of the same two atoms in a short cell).  Adds the force
+// to f and the unhalved energy to e.
 __device__ void nb_pbc_atom(const BatchDev& B, const SysInfo& I, int i, V3 pi, float4 ri, V3& f, double& e) {
-    const int N = I.natoms, W = (N + 31) >> 5;
+    const int N = I.natoms;
     const float kQi = COULOMB_K * ri.z;
     const float4* sh = B.pbcshifts + I.sh0;
-    auto nearest = [&](V3 d0) {
-        int smin = 0;
-        float r2min = INFINITY;
-        for (int s = 0; s < I.npbc; s++) {
-            const V3 d = d0 + xyz(sh[s]);
-            const float r2 = dot(d, d);
-            if (r2 < r2min) {
-                r2min = r2;
-                smin = s;
-            }
-        }
-        return smin;
-    };
+    const int4 ng = B.neighs[I.a0 + i];
+    const int4 nc = B.neighCell[I.a0 + i];
     for (int j = 0; j < N; j++) {
         if (j == i) continue;
         const V3 d0 = xyz(B.atoms[I.v0 + j]) - pi;
         const float4 rj = B.REQs[I.a0 + j];
-        const bool ex = (B.excl[I.mask0 + i * W + (j >> 5)] >> (j & 31)) & 1u;
-        const int skip = ex ? nearest(d0) : -1;
         for (int s = 0; s < I.npbc; s++) {
-            if (s == skip) continue;
+            if ((j == ng.x && s == nc.x) || (j == ng.y && s == nc.y) || (j == ng.z && s == nc.z) ||
+                (j == ng.w && s == nc.w))
+                continue;
             const V3 d = d0 + xyz(sh[s]);
             float fr, ep;
             lj_pair<true, true>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);
@@ -53,9 +44,13 @@ __device__ void nb_pbc_atom(const BatchDev& B, const SysInfo& I, int i, V3 pi, f
     }
     for (int c = 0; c < I.ncorr; c++) {
         const PairCorrDev pc = B.corr[I.corr0 + c];
-        if (pc.i != i) continue;
-        const V3 d0 = xyz(B.atoms[I.v0 + pc.j]) - pi;
-        const V3 d = d0 + xyz(sh[nearest(d0)]);
+        if (pc.i != i || pc.path < 0) continue;
+        const int node = pc.path >> 4, ki = (pc.path >> 2) & 3, kj = pc.path & 3;
+        const int* cell = reinterpret_cast<const int*>(B.neighCell + I.a0 + node);
+        const int cj = cell[kj], ci = cell[ki];  // -1 only for non-finite positions at packing
+        V3 d = xyz(B.atoms[I.v0 + pc.j]) - pi;
+        if (cj >= 0) d += xyz(sh[cj]);
+        if (ci >= 0) d -= xyz(sh[ci]);
         const float4 rj = B.REQs[I.a0 + pc.j];
         float fr, ep;
         lj_pair<true, true>(d, ri.x + rj.x, ri.y * rj.y, kQi * rj.z, ri.w * rj.w, fr, ep);

@@ -168,7 +168,32 @@ void World::rebuild_exclusions() {
         std::vector<uint32_t> rot;
         if (!fc::rotate_exclusions(m.natoms, mask, W, rot)) ring_ok = false;
         excl_rot.insert(excl_rot.end(), rot.begin(), rot.end());
-        for (auto& c : pc) corr.push_back({c.i, c.j, c.w, 0.f});
+        if (!m.pbc) {
+            for (auto& c : pc) corr.push_back({c.i, c.j, c.w, -1});
+            continue;
+        }
+        // Periodic molecule: the image loop skips a bonded neighbour only at the
+        // image of the bond (NBFF::evalLJQs_ng4_PBC_atom_omp, NBFF.h:575-620), and
+        // every 1-3 pair is subtracted once per angle path at the image reached
+        // through the bonds (MMFFsp3_loc::eval_atom, MMFFsp3_loc.h:525-536), which
+        // may be an image outside the summed range or the image of a 1-2 bond of
+        // the same two atoms in a short cell.  One directed entry per path and
+        // end atom (w = -1, halved with the rest of the pair energy); `path`
+        // names the node and the two slots, whose neighbour images give the shift.
+        info[s].ncorr = 0;
+        if (!sub13) continue;
+        for (int c = 0; c < m.nnode; c++) {
+            const int* ng = m.neighs.data() + c * 4;
+            for (int a = 0; a < 3; a++) {
+                if (ng[a] < 0) break;
+                for (int b = a + 1; b < 4; b++) {
+                    if (ng[b] < 0) break;
+                    corr.push_back({ng[a], ng[b], -1.f, c * 16 + a * 4 + b});
+                    corr.push_back({ng[b], ng[a], -1.f, c * 16 + b * 4 + a});
+                    info[s].ncorr += 2;
+                }
+            }
+        }
     }
 }
 
@@ -448,6 +473,7 @@ bool World::upload_all(bool bParams, bool bForces, bool bVel) {
         d_ebond.alloc(std::max<size_t>((size_t)ntotM, 1));
         d_enb.alloc(std::max<size_t>((size_t)ntotA, 1));
         d_fire.alloc(std::max(nSys, 1));
+        d_stall.alloc(std::max(nSys, 1));
         d_nsteps.upload(nsteps, stream);
         d_status.upload(status, stream);
         d_Esys.upload(Esys, stream);
@@ -534,6 +560,7 @@ fcd::BatchDev World::batch_dev() {
     B.Esys = d_Esys.p;
     B.Fmax = d_Fmax.p;
     B.fire = d_fire.p;
+    B.stall = d_stall.p;
     B.comp = d_comp.p;
     B.ncomp = d_ncomp.p;
     B.elem = d_elem.p;
@@ -580,6 +607,7 @@ fcd::RunParams World::run_params(double Fconv, int nstep_max) {
     P.pc_gstar = pc_gstar;
     P.pc_grigid = pc_grigid;
     P.pc_every = std::max(pc_every, 1);
+    P.pc_stall = std::max(pc_stall, 0);
     if (!surf_pos.empty()) {
         P.surf_pos = d_surf_pos.p;
         P.surf_REQ = d_surf_REQ.p;

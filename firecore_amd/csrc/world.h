@@ -67,6 +67,7 @@ struct World {
     int pc_mode = 0;
     float pc_gstar = 3.f, pc_grigid = 63.f;
     int pc_every = 8;
+    int pc_stall = 0;  // learned form: proposal points without a new best max|F| before plain FIRE
     // proposal network parameters: FP32 master copy in the packing order
     // W1[64][16] b1[64] W2[64][128] b2[64] W3[64][128] b3[64] W4[2][64] b4[2]
     std::vector<float> mlp_theta;
@@ -152,7 +153,7 @@ struct World {
     DevArr<float4> d_pcforce;
     DevArr<float4> d_gridcache;  // per-atom GridFF window cache (persistent kernel)
     DevArr<float4> d_atoms, d_atoms0, d_aforce, d_avel, d_REQs, d_PLQs, d_apars, d_bLs, d_bKs, d_Ksp, d_Kpp, d_fneigh,
-        d_fire, d_grid;
+        d_fire, d_stall, d_grid;
     DevArr<int4> d_neighs, d_bkNeighs, d_neighCell;
     DevArr<float4> d_pbcshifts;
     int n_periodic = 0;  // periodic molecules in the batch (relaxed by the split kernels)

@@ -577,6 +577,27 @@ def get_proposal_weights():
     return th
 
 
+def set_proposal_stall(windows):
+    """FIRE fallback of the learned form after `windows` proposal points in a
+    row without a new lowest max|F| (0 = never)."""
+    L = lib()
+    L.set_proposal_stall.argtypes = [c_int]
+    L.set_proposal_stall.restype = c_int
+    if L.set_proposal_stall(int(windows)) != 0:
+        raise ValueError("set_proposal_stall: windows must be >= 0")
+
+
+def proposal_fallback():
+    """Per system: True where the last learned relaxation fell back to plain FIRE."""
+    L = lib()
+    L.get_proposal_fallback.argtypes = [c_void_p]
+    L.get_proposal_fallback.restype = c_int
+    out = np.zeros(batch_sizes()["nsys"], np.int32)
+    if L.get_proposal_fallback(out.ctypes.data) != 0:
+        raise RuntimeError("get_proposal_fallback failed")
+    return out.astype(bool)
+
+
 def set_proposal_every(every):
     L = lib()
     L.set_proposal_every.argtypes = [c_int]

@@ -306,6 +306,13 @@ int get_proposal_nparam(void);
 int set_proposal_weights(const float* theta, int n);
 int get_proposal_weights(float* theta, int n);
 int set_proposal_every(int every);
+// FIRE fallback of the learned form: after `windows` proposal points in a row
+// without a new lowest max|F| a system continues with plain FIRE (zero gains,
+// no more network evaluations); 0 = never.  No counterpart in the reference
+// (which has no learned form).  0 on success.
+int set_proposal_stall(int windows);
+// Per system, 1 when the last learned relaxation fell back to plain FIRE.
+int get_proposal_fallback(int* out);
 /* trajectory snapshots of the next relax() (every `every` steps, at most n) */
 int record_snapshots(int every, int n);
 int load_snapshots(const int* idx);

@@ -112,6 +112,27 @@ def test_build_matches_reference_live(ref, name):
     assert (ml is None) == (rl is None)
 
 
+@pytest.mark.parametrize("name", PERIODIC)
+def test_oracle_matches_reference_on_built_periodic(orc, ref, name):
+    """The oracle's periodic force field against the reference's on the built
+    molecules; polydiacetylene's 5 A cell makes atoms both 1-2 and 1-3
+    partners through different images, and puts 1-3 paths at images outside
+    the summed range."""
+    from firecore_amd import native
+    m, lv = build(name)
+    sh, nc = native.make_neigh_cells(m["apos"][:m["natoms"]], m["neighs"], lv, (1, 1, 0))
+    orc.set_pbc(m["natoms"], sh, nc)
+    ref.set_pbc_ref(lv, (1, 1, 0))
+    try:
+        fo, _, _, Eo = orc.eval(m)
+        fr, _, _, Er = ref.eval(m)
+    finally:
+        orc.set_pbc()
+        ref.set_pbc_ref()
+    assert np.abs(fo - fr).max() < 1e-10 * np.abs(fr).max()
+    assert abs(Eo - Er) < 1e-10 * (1 + abs(Er))
+
+
 @pytest.mark.parametrize("text,what", [
     ("2\n\nC 0 0 0 0\nC 10 0 0 0\n", "atom without bonds"),
     ("3\n\nC 0 0 0 0\nH 0.9 0 0 0\nC 1.8 0 0 0\n", "capping atom with two bonds"),
@@ -136,6 +157,35 @@ def test_build_missing_file_raises():
 
 
 # ------------------------------------------------------------------ GPU
+@pytest.mark.gpu
+def test_init_xyz_forces_match_oracle(orc):
+    """Device forces and energies of the built periodic molecules at their
+    file geometry against the FP64 oracle (max|dF| <= 1e-4 + 5e-6 max|F|
+    eV/A, |dE| <= 1e-4 + 1e-6 |E| eV)."""
+    from firecore_amd import native
+    mols, cells = [], []
+    for name in PERIODIC:
+        m, lv = build(name)
+        mols.append(m)
+        cells.append(lv)
+    native.init_systems(mols)
+    for s, lv in enumerate(cells):
+        native.set_lattice(s, lv, (1, 1, 0))
+    P("case: built periodic molecules, forces", PERIODIC)
+    native.eval_forces()
+    B = native.Batch()
+    for s, (m, lv) in enumerate(zip(mols, cells)):
+        sh, nc = native.make_neigh_cells(m["apos"][:m["natoms"]], m["neighs"], lv, (1, 1, 0))
+        orc.set_pbc(m["natoms"], sh, nc)
+        try:
+            fo, _, _, Eo = orc.eval(B.system(s))
+        finally:
+            orc.set_pbc()
+        d = np.abs(B.forces(s) - fo).max()
+        assert d <= 1e-4 + 5e-6 * np.abs(fo).max(), (PERIODIC[s], d)
+        assert abs(B.E[s] - Eo) <= 1e-4 + 1e-6 * abs(Eo), (PERIODIC[s], B.E[s], Eo)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["PTCDA", "guanine", "nHexadecan_dicarboxylic", "polydiacetylene"])
 def test_init_xyz_relax(orc, name):
