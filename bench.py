@@ -100,12 +100,12 @@ def host_info():
     return dict(model=model, nproc=os.cpu_count() or 1, affinity=affinity)
 
 
-def cpu_baseline(native, grid, nsample, nthreads, reps=5):
+def cpu_baseline(native, grid, nsample, nthreads, reps=5, nsys=NSYS):
     """FP64 oracle of the reference CPU path (plain FIRE, one system per
     thread, dynamic schedule) on an evenly spaced sample of the workload:
     one warm-up run, then the median of `reps` timed runs."""
     import oracle
-    idx = np.linspace(0, NSYS - 1, nsample).astype(int)
+    idx = np.linspace(0, nsys - 1, nsample).astype(int)
     systems = [native.gen_workload_system(1, int(i), SEED0, NMIN, NMAX, 0.1, grid) for i in idx]
     fire = tuple(FIRE) + (FLIMIT,)
     rates, convs = [], []
@@ -120,27 +120,29 @@ def cpu_baseline(native, grid, nsample, nthreads, reps=5):
         log("cpu baseline run %d: %d converged in %.2f s" % (r, conv, dt))
     h = host_info()
     return dict(value=float(np.median(rates)), unit="relaxations/s", cores=nthreads, kind="port",
-                sample="%d of the 4096 systems (evenly spaced indices), FP64 oracle of the reference CPU path, "
+                sample="%d of the %d systems (evenly spaced indices), FP64 oracle of the reference CPU path, "
                        "plain FIRE, same limits; %d converged; median of %d runs after 1 warm-up (%.2f..%.2f/s)"
-                       % (nsample, convs[0], reps, min(rates), max(rates)),
+                       % (nsample, nsys, convs[0], reps, min(rates), max(rates)),
                 host=h["model"], host_nproc=h["nproc"], host_affinity=h["affinity"],
                 threads_note="one system per thread on %d threads: this box's CPU share per GPU" % nthreads)
 
 
-def measure(native, sync, steps, warmup, tag, nstep_max=NSTEP_MAX):
+def measure(native, sync, steps, warmup, tag, nstep_max=NSTEP_MAX, path=0):
     """Time `steps` relaxation passes after `warmup` untimed ones.  The queue
     order comes from the autotuned cost model (no step counts of earlier
-    passes), so every pass, the first included, schedules the same way."""
+    passes), so every pass, the first included, schedules the same way.
+    path 0: the persistent kernel; 1: the split kernels (one launch per phase
+    and step)."""
     first_ms = None
     for w in range(warmup):
-        native.relax(nstep_max, FCONV, 0, True, False)
+        native.relax(nstep_max, FCONV, path, True, False)
         log("%s warmup %d: %.1f ms" % (tag, w, native.last_ms()))
         first_ms = native.last_ms() if first_ms is None else first_ms
     sync()
     t = time.perf_counter()
     dev_ms, class_ms = [], []
     for k in range(steps):
-        native.relax(nstep_max, FCONV, 0, True, False)
+        native.relax(nstep_max, FCONV, path, True, False)
         dev_ms.append(native.last_ms())
         class_ms.append([c["ms"] for c in native.class_info()])
         log("%s step %d: %.1f ms" % (tag, k, dev_ms[-1]))
@@ -200,9 +202,15 @@ def roofline(r, learned, every=PROP_EVERY):
         if rec:
             traffic = rec["bytes_per_launch"]
             hbm_frac = round(traffic / (dom_ms * 1e-3) / (PEAK_HBM_GBS * 1e9), 4)
+    # the class launches of a pass run concurrently and share the device, so the
+    # dominant launch's rate understates the device's: the whole pass as well
+    pass_tflops = float(np.sum(flops)) / (r["dev_ms"] * 1e-3) / 1e12
     return dict(bound="mfma", achieved=round(tflops, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                 frac=round(tflops / PEAK_FP32_TFLOPS, 4), traffic=traffic,
-                pipe="FP32 vector ALU (no matrix instructions in this kernel); peak = the FP32 figure",
+                pass_flop=float(np.sum(flops)), pass_ms=round(r["dev_ms"], 3), pass_achieved=round(pass_tflops, 3),
+                pass_frac=round(pass_tflops / PEAK_FP32_TFLOPS, 4),
+                pipe="FP32 vector ALU (the step loop; the learned form's in-kernel network adds BF16 MFMA "
+                     "work of under 1 % of the launch's cycles); peak = the FP32 figure",
                 kernel=name, kernel_ms=round(dom_ms, 3), kernel_systems=classes[dom]["nsys"],
                 algorithmic_flop=flops[dom], hbm_frac=hbm_frac, grid_gather_bytes=gbytes[dom])
 
@@ -257,6 +265,13 @@ def main():
     ap.add_argument("--forms", default="plain,learned,constant")
     ap.add_argument("--no-screening", action="store_true")
     ap.add_argument("--screening-systems", type=int, default=SCREEN_NSYS)
+    ap.add_argument("--path", choices=("persistent", "split"), default="persistent",
+                    help="relaxation kernels: the persistent kernel (default) or the split per-phase kernels "
+                         "(a profiling mode: no launch plan, no roofline)")
+    ap.add_argument("--systems", type=int, default=NSYS, help="systems of the workload (default %d)" % NSYS)
+    ap.add_argument("--plans-out", help="write the calibrated launch plans (JSON) to this file")
+    ap.add_argument("--plans-in", help="use the launch plans of this file instead of calibrating (counter runs, "
+                                       "which serialise the launches and would calibrate differently)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="copy a 256 MiB buffer once first (the known-size kernel that scales memory counters)")
     args = ap.parse_args()
@@ -287,26 +302,36 @@ def main():
     if meta is None:
         forms = [f for f in forms if f != "learned"]
     # launch plans, tuned per form on systems of seeds disjoint from the benchmark's
-    plans = {}
+    path = 1 if args.path == "split" else 0
+    plans = json.load(open(args.plans_in)) if args.plans_in else {}
     for f in forms:
+        if path == 1:
+            plans[f] = None
+            continue
+        if f in plans:
+            continue
         plans[f] = autotune.calibrate(form_setup(native, f, meta), kind=1, nsys=1024, seed0=90001, nmin=NMIN,
                                       nmax=NMAX, nstep_max=NSTEP_MAX, log=log)
+    if args.plans_out:
+        with open(args.plans_out, "w") as fh:
+            json.dump(plans, fh)
     t0 = time.perf_counter()
-    native.init_workload(1, NSYS, SEED0, NMIN, NMAX, 0.1, "synth:NaCl", 0.1)
+    native.init_workload(1, args.systems, SEED0, NMIN, NMAX, 0.1, "synth:NaCl", 0.1)
     log("init %.1f s" % (time.perf_counter() - t0))
     grid = native.grid_info()
     res = {}
     for f in forms:
         form_setup(native, f, meta)()
-        autotune.apply(plans[f])
-        res[f] = measure(native, sync, args.steps, args.warmup, f)
+        if plans[f]:
+            autotune.apply(plans[f])
+        res[f] = measure(native, sync, args.steps, args.warmup, f, path=path)
     native.set_precond(0)
 
     head = "learned" if "learned" in res else ("plain" if "plain" in res else list(res)[0])
     r = res[head]
-    config = {"workload": "4096 uneven molecules N~U[16,256] on NaCl(001) GridFF 56x56x200, FIRE to "
-                          "max|f|<1e-3 eV/A, <=10000 steps",
-              "form": head, "systems": NSYS, "converged": r["conv"],
+    config = {"workload": "%d uneven molecules N~U[16,256] on NaCl(001) GridFF 56x56x200, FIRE to "
+                          "max|f|<1e-3 eV/A, <=10000 steps" % args.systems,
+              "form": head, "systems": args.systems, "converged": r["conv"],
               "max_step": int((r["status"] == native.ST_MAXSTEP).sum()),
               "nonfinite": int((r["status"] == native.ST_NAN).sum()),
               "mean_steps": round(float(r["nsteps"].mean()), 1),
@@ -326,13 +351,16 @@ def main():
     if meta:
         config["proposal"] = "every %d steps; gain ceilings star %g, molecule %g; trained on %d pairs from seeds %s" % (
             int(meta.get("every", PROP_EVERY)), meta["gstar"], meta["grigid"], meta["pairs"], meta["seeds"])
-    rl = roofline(r, head == "learned", int(meta.get("every", PROP_EVERY)) if meta else PROP_EVERY)
+    if path == 1:
+        config["path"] = "split kernels (profiling mode)"
+    rl = roofline(r, head == "learned", int(meta.get("every", PROP_EVERY)) if meta else PROP_EVERY) if path == 0 \
+        else None
     if not args.no_screening and meta:
         config["screening"] = screening_leg(native, sync, autotune, meta, args.screening_systems, 1)
     native.set_precond(0)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(native, grid, args.cpu_sample, min(host_info()["affinity"], 16))
+        cpu = cpu_baseline(native, grid, args.cpu_sample, min(host_info()["affinity"], 16), nsys=args.systems)
         log("cpu baseline: %s" % cpu)
     out = {
         "metric": "relaxations/sec to Fmax<1e-3 eV/Å, 4096 mixed systems; learned vs plain",

@@ -35,6 +35,18 @@ def fit_steps(N, steps):
     return c0, c1
 
 
+# an alternative replaces the default setting only when it is at least this
+# much faster: differences below it are within the run-to-run spread of a
+# calibration pass, and a plan that flips between runs changes which launch
+# dominates the benchmark pass
+MARGIN = 0.015
+
+
+def _pick(times, default):
+    best = min(times, key=times.get)
+    return best if times[best] < times[default] * (1.0 - MARGIN) else default
+
+
 def _time_pass(nstep_max, fconv, reps=2):
     ms = []
     for _ in range(reps):
@@ -64,13 +76,13 @@ def calibrate(setup_form, kind=1, nsys=512, seed0=90001, nmin=16, nmax=256, surf
     for sched in (1, 2, 0):
         native.set_sched(sched)
         trials["sched%d" % sched] = _time_pass(nstep_max, fconv)
-    best_sched = int(min(trials, key=trials.get)[5:])
+    best_sched = int(_pick(trials, "sched1")[5:])
     native.set_sched(best_sched)
     widths = {}
     for mask in (0xF, 0xB, 0xD, 0x9):  # bit c enables width 64*(c+1); 256 always on
         native.set_class_mask(mask)
         widths[mask] = _time_pass(nstep_max, fconv)
-    best_mask = int(min(widths, key=widths.get))
+    best_mask = int(_pick(widths, 0xF))
     native.set_class_mask(best_mask)
     plan = dict(cost_c0=round(c0, 3), cost_c1=round(c1, 4), sched=best_sched, class_mask=best_mask,
                 widths=[64 * (c + 1) for c in range(4) if (best_mask >> c) & 1],

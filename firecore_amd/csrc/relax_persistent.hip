@@ -91,8 +91,16 @@ struct Queues {
     unsigned short* hslot;  // learned gains: global activation rows, one slot per workgroup of this launch
 };
 
+// minimum resident waves per SIMD the register allocation must allow
+#ifndef FC_WAVES_PER_SIMD
+#define FC_WAVES_PER_SIMD 3
+#endif
+#ifndef FC_WAVES_PER_SIMD_LEARNED
+#define FC_WAVES_PER_SIMD_LEARNED FC_WAVES_PER_SIMD
+#endif
 template <int BLOCK, bool HB, int PC>
-__global__ void __launch_bounds__(BLOCK, 3) relax_persistent_kernel(BatchDev B, RunParams P, Queues Q, int cls,
+__global__ void __launch_bounds__(BLOCK, PC == 2 ? FC_WAVES_PER_SIMD_LEARNED : FC_WAVES_PER_SIMD)
+relax_persistent_kernel(BatchDev B, RunParams P, Queues Q, int cls,
                                                                  LdsLayout L) {
     extern __shared__ float4 lds_dyn[];
     char* lds = reinterpret_cast<char*>(lds_dyn);
