@@ -22,7 +22,7 @@ HOST_FLAGS = ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
 
 HIP_SOURCES = ["relax_persistent.hip", "split_kernels.hip", "sample_kernels.hip", "proposal_train.hip",
                "gridff_gen.hip"]
-CPP_SOURCES = ["molgen.cpp", "topology.cpp", "gridff_host.cpp", "builder.cpp", "world.cpp", "train.cpp", "capi.cpp"]
+CPP_SOURCES = ["molgen.cpp", "topology.cpp", "gridff_host.cpp", "builder.cpp", "database.cpp", "world.cpp", "train.cpp", "capi.cpp"]
 
 
 def _compile(src):

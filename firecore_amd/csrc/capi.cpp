@@ -951,6 +951,60 @@ int get_minima(int isys, double* Es, double* poss, int maxn) {
     return found;
 }
 
+// Minima database of the exploring mode (MolecularDatabase, filled by explore()
+// and by run() after init(nExplore > 0) while setSwitches_multi(...,
+// bSaveToDatabase = 1) is on; the reference appends new members to gopt.xyz).
+// i < 0: the number of members.  Else member i's atom count, with its energy,
+// system, occurrence count and positions (natoms x 3, at most maxatoms rows)
+// written where the pointers are not null; -1 when i is out of range.
+int get_database_member(int i, double* E, int* isys, int* occurrence, double* pos, int maxatoms) {
+    const fc::MinimaDB& D = world().db;
+    if (i < 0) return (int)D.members.size();
+    if (i >= (int)D.members.size()) return -1;
+    const fc::MinimaDB::Member& m = D.members[i];
+    if (E) *E = m.E;
+    if (isys) *isys = m.isys;
+    if (occurrence) *occurrence = m.occurrence;
+    if (pos) std::memcpy(pos, m.pos.data(), sizeof(double) * 3 * std::min(m.natoms, std::max(maxatoms, 0)));
+    return m.natoms;
+}
+// Offer a structure of n atoms (positions n x 3, a type label per atom) to
+// the database: the index of the member it duplicates, or -1 when it became a
+// new member.  Host only.
+int database_add(int n, const double* pos, const int* types, double E) {
+    if (n < 1 || !pos || !types) return -2;
+    return world().db.add_if_new(n, pos, types, E, -1);
+}
+void clear_database() { world().db.clear(); }
+// Descriptor of a structure (sqrt of the inertia tensor's eigenvalues about the
+// centroid, ascending, then n; the eigenvalues only for n >= 3): its length.
+int database_descriptor(int n, const double* pos, double* d4) { return fc::db_descriptor(n, pos, d4); }
+// Distance the database compares with 0.1 n: sqrt of the sum over y's atoms of
+// the squared distance to the nearest same-type atom of x, after the optimal
+// superposition of the centred x on the centred y.
+double database_distance(int n, const double* x, const int* tx, const double* y, const int* ty) {
+    return fc::db_distance(n, x, tx, y, ty);
+}
+// The members as xyz frames (element symbols of the type labels C, N, O, H
+// codes 0-3, else X), comment "# <index> E <energy> occurrence <count> isys <s>".
+int save_database_xyz(const char* fname) {
+    const fc::MinimaDB& D = world().db;
+    FILE* f = std::fopen(fname, "w");
+    if (!f) return -1;
+    static const char* sym[4] = {"C", "N", "O", "H"};
+    for (size_t i = 0; i < D.members.size(); i++) {
+        const fc::MinimaDB::Member& m = D.members[i];
+        std::fprintf(f, "%d\n# %zu E %.10g occurrence %d isys %d\n", m.natoms, i, m.E, m.occurrence, m.isys);
+        for (int a = 0; a < m.natoms; a++) {
+            const int t = m.types[a];
+            std::fprintf(f, "%s %.8f %.8f %.8f\n", (t >= 0 && t < 4) ? sym[t] : "X", m.pos[a * 3], m.pos[a * 3 + 1],
+                         m.pos[a * 3 + 2]);
+        }
+    }
+    std::fclose(f);
+    return (int)D.members.size();
+}
+
 // Periodic molecules (MMFFsp3_loc with bPBC: image sums of the non-bonded
 // terms, bonded neighbours at their nearest image, MMFFsp3_loc.h:325-329,
 // NBFF.h:575-620): lattice rows lvec9 (a, b, c) and image range nPBC3 (null =

@@ -146,6 +146,33 @@ struct TypeTables {
 bool build_from_xyz(const std::string& fname, const TypeTables& T, bool& periodic, double* lvec9, MolSys& out,
                     std::string& err);
 
+// ---------------- minima database of the exploring mode (database.cpp) ----------------
+// descriptor of a structure (principal axes + atom count): returns its length
+int db_descriptor(int n, const double* pos, double* d4);
+// sqrt(sum over y's atoms of the squared distance to the nearest same-type atom
+// of x after the optimal superposition of the centred x on the centred y)
+double db_distance(int n, const double* x, const int* tx, const double* y, const int* ty);
+struct MinimaDB {
+    static constexpr double DESC_TOL = 5.0;  // descriptor distance of candidates
+    static constexpr double DIST_TOL = 0.1;  // db_distance / natoms of duplicates
+    struct Member {
+        int natoms = 0, ndesc = 0, isys = -1, occurrence = 0;
+        double desc[4] = {0, 0, 0, 0};
+        double E = 0;
+        std::vector<double> pos;
+        std::vector<int> types;
+    };
+    std::vector<Member> members;
+    long total = 0;  // structures offered
+    // index of the member the structure duplicates (its count raised), or -1
+    // when it was added as a new member
+    int add_if_new(int n, const double* pos, const int* types, double E, int isys);
+    void clear() {
+        members.clear();
+        total = 0;
+    }
+};
+
 // ---------------- GridFF (gridff_host.cpp) ----------------
 struct GridFFHost {
     int nx = 0, ny = 0, nz = 0;

@@ -1126,7 +1126,30 @@ int World::explore(int nstep_max, double Fconv, double T, double gamma, int n_ex
         tot += xs[s].z;
     }
     minpos_kept = keep_positions;
+    if (bSaveToDatabase && keep_positions) feed_database();
     return tot;
+}
+
+// MolWorld_sp3_multi::evalVFs with bSaveToDatabase (MolWorld_sp3_multi.h:863-879)
+// offers every converged relaxation of an exploring run to the database as it
+// happens; here the recorded minima are offered after the run, by record index
+// and, within one index, in system order.
+void World::feed_database() {
+    std::vector<double> pos;
+    for (int m = 0; m < xkmax; m++)
+        for (int s = 0; s < nSys; s++) {
+            if (m >= nmin[s]) continue;
+            const fcd::SysInfo& I = info[s];
+            const int N = I.natoms;
+            pos.resize((size_t)N * 3);
+            for (int i = 0; i < N; i++) {
+                const float4 p = minpos[(size_t)m * ntotV + I.v0 + i];
+                pos[i * 3] = p.x;
+                pos[i * 3 + 1] = p.y;
+                pos[i * 3 + 2] = p.z;
+            }
+            db.add_if_new(N, pos.data(), mols[s].elem.data(), minE[(size_t)s * xkmax + m], s);
+        }
 }
 
 bool World::eval_forces() {

@@ -235,6 +235,9 @@ struct World {
     std::vector<int> nmin;       // minima found per system (may exceed xkmax)
     int xkmax = 0;
     bool minpos_kept = false;
+    // minima database (filled by explore() while bSaveToDatabase is on)
+    fc::MinimaDB db;
+    void feed_database();
     // exploring parameters of init() (T, gamma, nExplore, nRelax); run() explores when nExplore > 0
     double go_T = -1, go_gamma = -1;
     int go_nExplore = 0, go_nRelax = 0;

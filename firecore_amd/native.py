@@ -445,6 +445,72 @@ def minima(isys, maxn=16):
     return n, E[:k], P[:k]
 
 
+def database():
+    """Members of the minima database: list of dicts (natoms, E, isys,
+    occurrence, pos (natoms, 3))."""
+    L = lib()
+    L.get_database_member.argtypes = [c_int, c_double_p, c_int_p, c_int_p, c_double_p, c_int]
+    L.get_database_member.restype = c_int
+    n = L.get_database_member(-1, None, None, None, None, 0)
+    out = []
+    for i in range(n):
+        na = L.get_database_member(i, None, None, None, None, 0)
+        E = np.zeros(1)
+        s = np.zeros(1, np.int32)
+        occ = np.zeros(1, np.int32)
+        P = np.zeros((na, 3))
+        L.get_database_member(i, _p(E), _p(s, c_int_p), _p(occ, c_int_p), _p(P), na)
+        out.append(dict(natoms=na, E=float(E[0]), isys=int(s[0]), occurrence=int(occ[0]), pos=P))
+    return out
+
+
+def database_add(pos, types, E=0.0):
+    """Offer a structure to the minima database: index of the member it
+    duplicates, or -1 when it was added."""
+    L = lib()
+    L.database_add.argtypes = [c_int, c_double_p, c_int_p, c_double]
+    L.database_add.restype = c_int
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    types = np.ascontiguousarray(types, np.int32)
+    r = L.database_add(len(pos), _p(pos), _p(types, c_int_p), float(E))
+    if r < -1:
+        raise ValueError("database_add: invalid structure")
+    return r
+
+
+def clear_database():
+    lib().clear_database()
+
+
+def database_descriptor(pos):
+    L = lib()
+    L.database_descriptor.argtypes = [c_int, c_double_p, c_double_p]
+    L.database_descriptor.restype = c_int
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    d = np.zeros(4)
+    n = L.database_descriptor(len(pos), _p(pos), _p(d))
+    return d[:n]
+
+
+def database_distance(x, tx, y, ty):
+    L = lib()
+    L.database_distance.argtypes = [c_int, c_double_p, c_int_p, c_double_p, c_int_p]
+    L.database_distance.restype = ctypes.c_double
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+    y = np.ascontiguousarray(y, np.float64).reshape(-1, 3)
+    tx = np.ascontiguousarray(tx, np.int32)
+    ty = np.ascontiguousarray(ty, np.int32)
+    return L.database_distance(len(x), _p(x), _p(tx, c_int_p), _p(y), _p(ty, c_int_p))
+
+
+def set_save_to_database(on=True):
+    """setSwitches_multi(..., bSaveToDatabase): explore() offers its converged
+    minima to the database while on."""
+    L = lib()
+    L.setSwitches_multi.argtypes = [c_int] * 9
+    L.setSwitches_multi(0, 0, 0, 0, 0, 0, 0, 0, 1 if on else -1)
+
+
 def last_ms():
     return lib().get_last_ms()
 

@@ -244,6 +244,28 @@ int explore(int nstepMax, double Fconv, double T, double gamma, int nExplore, in
  * recorded, positions (natoms+nnode rows of x, y, z each); at most maxn.
  * Returns the number found (may exceed kmax), -1 on error. */
 int get_minima(int isys, double* Es, double* poss, int maxn);
+/* Minima database of the exploring mode, replaces MolecularDatabase
+ * (MolecularDatabase.h:430-474, filled in MolWorld_sp3_multi.h:863-879 while
+ * setSwitches_multi(..., bSaveToDatabase) is on; the reference writes new
+ * members to gopt.xyz): the converged minima of explore()/run() are offered
+ * to it; a structure duplicates a member when their descriptors (principal
+ * axes, atom count) differ by < 5 and, optimally superposed, the root of the
+ * summed squared nearest-same-type distances is < 0.1 per atom.
+ * get_database_member(i<0) returns the member count; else member i's atom
+ * count, with energy, system, occurrence count and positions (natoms x 3,
+ * at most maxatoms rows) written where not null; -1 when out of range. */
+int get_database_member(int i, double* E, int* isys, int* occurrence, double* pos, int maxatoms);
+/* Offer a structure of n atoms (positions n x 3, a type label per atom): the
+ * member index it duplicates, or -1 when added.  Host only. */
+int database_add(int n, const double* pos, const int* types, double E);
+void clear_database(void);
+/* Descriptor (sqrt of the ascending inertia eigenvalues about the centroid
+ * for n >= 3, then n); returns its length (at most 4). */
+int database_descriptor(int n, const double* pos, double* d4);
+/* The distance the database compares with 0.1 * n (see above). */
+double database_distance(int n, const double* x, const int* tx, const double* y, const int* ty);
+/* Members as xyz frames; returns the member count, -1 when the file cannot be written. */
+int save_database_xyz(const char* fname);
 /* Periodic molecules (MMFFsp3_loc with bPBC: the non-bonded sum runs over the
  * image shifts of evalPBCshifts, MolWorld_sp3.h:756-765, as
  * NBFF::evalLJQs_ng4_PBC_atom_omp, NBFF.h:575-620; bonded neighbours sit at the

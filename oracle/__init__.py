@@ -197,6 +197,40 @@ class Backend:
         f(ctypes.c_int(N), ctypes.c_int(M), _p(apos), _p(ng, _ip), _p(lv), _p(npb, _ip), _p(nc, _ip), _p(sh))
         return sh, nc
 
+    def db_reset(self):
+        """Reference build: a fresh MolecularDatabase with its default descriptors."""
+        self.lib.ref_db_reset.restype = None
+        self.lib.ref_db_reset()
+
+    def db_add(self, pos, types):
+        """Reference build: MolecularDatabase::addIfNewDescriptor of a
+        structure (-1 = new member, else the index it duplicates)."""
+        f = self.lib.ref_db_add
+        f.restype = ctypes.c_int
+        p = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(types, np.int32)
+        return f(ctypes.c_int(len(p)), _p(p), _p(t, _ip))
+
+    def db_count(self):
+        self.lib.ref_db_count.restype = ctypes.c_int
+        return self.lib.ref_db_count()
+
+    def db_descriptor(self, pos):
+        f = self.lib.ref_db_descriptor
+        f.restype = ctypes.c_int
+        p = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        out = np.zeros(8)
+        n = f(ctypes.c_int(len(p)), _p(p), _p(out))
+        return out[:n]
+
+    def db_distance(self, x, tx, y, ty):
+        f = self.lib.ref_db_distance
+        f.restype = ctypes.c_double
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        y = np.ascontiguousarray(y, np.float64).reshape(-1, 3)
+        return f(ctypes.c_int(len(x)), _p(x), _p(np.ascontiguousarray(tx, np.int32), _ip), _p(y),
+                 _p(np.ascontiguousarray(ty, np.int32), _ip))
+
     def build_xyz_ref(self, res_dir, xyz_noext, nPBC=(1, 1, 0), maxn=4096):
         """Reference build: the reference's own MolWorld_sp3::init of an .xyz
         molecule (run in res_dir, where its common_resources/ live); returns

@@ -468,4 +468,83 @@ void ref_gridff_points(int natoms, const double* apos, const double* REQs, const
     delete[] sc;
 }
 
+// ---- the reference's MolecularDatabase (MolecularDatabase.h) with its default
+// descriptors (principal axes of all atoms, number of atoms).  Structures are
+// kept alive for the database's lifetime (it stores copies, but the candidate's
+// arrays are read during addIfNewDescriptor).
+static MolecularDatabase* g_db = nullptr;
+
+// the arrays a structure was offered with, kept for the process lifetime
+static void db_bind(ForceField& ff, int n, const double* pos, const int* types) {
+    Vec3d* p = new Vec3d[n];
+    int* t = new int[n];
+    for (int i = 0; i < n; i++) {
+        p[i] = Vec3d{pos[i * 3], pos[i * 3 + 1], pos[i * 3 + 2]};
+        t[i] = types[i];
+    }
+    ff.natoms = n;
+    ff.apos = p;
+    ff.atypes = t;
+    ff.vapos = new Vec3d[n]();
+    ff.fapos = new Vec3d[n]();
+}
+
+void ref_db_reset() {
+    g_db = new MolecularDatabase();  // the previous one is left allocated (test process only)
+    Atoms dummy(3);
+    for (int i = 0; i < 3; i++) {
+        dummy.atypes[i] = 0;
+        dummy.apos[i] = Vec3d{(double)i, 0, 0};
+    }
+    g_db->setDescriptors(&dummy);  // default descriptor set; reads atypes[0..2] of its argument
+}
+
+// addIfNewDescriptor: -1 = added as a new member, else the index it duplicates
+int ref_db_add(int n, const double* pos, const int* types) {
+    if (!g_db) ref_db_reset();
+    ForceField ff;
+    db_bind(ff, n, pos, types);
+    const int r = g_db->addIfNewDescriptor(&ff);
+    ff.apos = nullptr;
+    ff.atypes = nullptr;
+    ff.vapos = nullptr;
+    ff.fapos = nullptr;
+    return r;
+}
+
+int ref_db_count() { return g_db ? g_db->getNMembers() : 0; }
+
+int ref_db_descriptor(int n, const double* pos, double* out) {
+    std::vector<int> t(n, 0);
+    Atoms a;
+    std::vector<Vec3d> p(n);
+    for (int i = 0; i < n; i++) p[i] = Vec3d{pos[i * 3], pos[i * 3 + 1], pos[i * 3 + 2]};
+    a.bind(n, t.data(), p.data());
+    Descriptor d;
+    d.setDescriptor(MolecularDatabaseDescriptor{principal_axes, -1}, &a);
+    d.setDescriptor(MolecularDatabaseDescriptor{number_of_atoms, -1}, &a);
+    const int nd = d.dimensionOfDescriptorSpace;
+    for (int k = 0; k < nd; k++) out[k] = d.descs[k];
+    a.bind(0, nullptr, nullptr);
+    return nd;
+}
+
+// computeDistance(x, y) of the database: positions of both are restored after
+double ref_db_distance(int n, const double* x, const int* tx, const double* y, const int* ty) {
+    if (!g_db) ref_db_reset();
+    std::vector<Vec3d> px(n), py(n);
+    std::vector<int> vx(tx, tx + n), vy(ty, ty + n);
+    for (int i = 0; i < n; i++) {
+        px[i] = Vec3d{x[i * 3], x[i * 3 + 1], x[i * 3 + 2]};
+        py[i] = Vec3d{y[i * 3], y[i * 3 + 1], y[i * 3 + 2]};
+    }
+    Atoms ax, ay;
+    ax.bind(n, vx.data(), px.data());
+    ay.bind(n, vy.data(), py.data());
+    const double d = g_db->computeDistance(&ax, &ay);
+    ax.bind(0, nullptr, nullptr);
+    ay.bind(0, nullptr, nullptr);
+    return d;
+}
+
 }  // extern "C"
