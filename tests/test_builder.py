@@ -227,3 +227,74 @@ def test_init_xyz_relax(orc, name):
             orc.set_pbc()
     finally:
         native.set_opt()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", [0, 1])
+def test_typed_trajectory_parity(orc, path):
+    """Reference molecules typed by the builder (sp2 pi-systems, N and O lone
+    pairs, up to 139 atoms), displaced by 0.1 A from their file geometry: the
+    device FIRE trajectory follows the FP64 oracle's over 40 steps within 2e-4 A
+    (persistent kernel and split kernels)."""
+    from conftest import FIRE_REF
+    from firecore_amd import native
+    rng = np.random.default_rng(5)
+    mols = []
+    for name in ("guanine", "Benzene_deriv", "nHexadecan_dicarboxylic", "N_2edge_assay_1"):
+        m, _ = build(name)
+        m["apos"] = np.array(m["apos"])
+        m["apos"][:m["natoms"]] += rng.normal(size=(m["natoms"], 3)) * 0.1
+        mols.append(m)
+    native.set_force_limit(0.0)
+    native.set_opt(*FIRE_REF)
+    native.init_systems(mols)
+    P("case: typed molecules, 40-step trajectory, path", path)
+    try:
+        native.relax(40, 1e-12, path, True, True)
+        B = native.Batch()
+        assert (np.array(B.status) == native.ST_MAXSTEP).all() and (np.array(B.nsteps) == 40).all()
+        for s, m in enumerate(mols):
+            _, _, _, traj = orc.relax(m, FIRE_REF, 1e-12, 40, 0, ntraj=40)
+            d = np.abs(B.positions(s) - traj[-1]).max()
+            assert d < 2e-4, (s, d)
+            assert np.abs(traj[-1] - m["apos"]).max() > 0.01  # the systems did move
+    finally:
+        native.set_opt()
+
+
+@pytest.mark.gpu
+def test_periodic_typed_trajectory_parity(orc):
+    """Periodic reference molecules (polydiacetylene's chain bonded across the
+    cell, PTCDA, BPBA), displaced by 0.05 A: 40 FIRE steps of the split kernels
+    against the FP64 oracle with the same image shifts and neighbour images."""
+    from conftest import FIRE_REF
+    from firecore_amd import native
+    rng = np.random.default_rng(6)
+    mols, cells = [], []
+    for name in ("polydiacetylene", "PTCDA", "BPBA"):
+        m, lv = build(name)
+        m["apos"] = np.array(m["apos"])
+        m["apos"][:m["natoms"]] += rng.normal(size=(m["natoms"], 3)) * 0.05
+        mols.append(m)
+        cells.append(lv)
+    native.set_force_limit(0.0)
+    native.set_opt(*FIRE_REF)
+    native.init_systems(mols)
+    for s, lv in enumerate(cells):
+        native.set_lattice(s, lv, (1, 1, 0))
+    P("case: periodic typed molecules, 40-step trajectory")
+    try:
+        native.relax(40, 1e-12, 1, True, True)
+        B = native.Batch()
+        assert (np.array(B.status) == native.ST_MAXSTEP).all()
+        for s, (m, lv) in enumerate(zip(mols, cells)):
+            sh, nc = native.make_neigh_cells(m["apos"][:m["natoms"]], m["neighs"], lv, (1, 1, 0))
+            orc.set_pbc(m["natoms"], sh, nc)
+            try:
+                _, _, _, traj = orc.relax(m, FIRE_REF, 1e-12, 40, 0, ntraj=40)
+            finally:
+                orc.set_pbc()
+            d = np.abs(B.positions(s) - traj[-1]).max()
+            assert d < 2e-4, (s, d)
+    finally:
+        native.set_opt()
