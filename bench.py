@@ -21,7 +21,7 @@ limit), inputs already resident in HBM.  `value` is the learned form.
 
 Before the timed passes each form's launch plan (queue-order cost model,
 scheduling mode, workgroup widths) is tuned by firecore_amd/autotune.py on
-1024 systems of seeds 90001.. (disjoint from the benchmark batch); the timed
+4096 systems of seeds 90001.. (disjoint from the benchmark batch); the timed
 passes never use step counts of earlier passes over the benchmark batch.
 A second leg relaxes the screening workload: 65,536 molecule pairs (8-64
 atoms each, seeds 20001..), learned form, at most 3000 steps.
@@ -42,6 +42,7 @@ sys.path.insert(0, ROOT)
 NSYS = 4096
 NMIN, NMAX = 16, 256
 SEED0 = 10001
+CALIB_SEED0 = 90001  # calibration batch of the launch plans (disjoint from SEED0..)
 FCONV = 1e-3
 NSTEP_MAX = 10000
 FIRE = (0.1, 0.02, 0.2, 1.1, 0.5, 0.8, 5)  # dt_max, dt_min, damp_max, finc, fdec, falpha, minLastNeg
@@ -304,14 +305,19 @@ def main():
     # launch plans, tuned per form on systems of seeds disjoint from the benchmark's
     path = 1 if args.path == "split" else 0
     plans = json.load(open(args.plans_in)) if args.plans_in else {}
+    # one calibration batch of the benchmark's size and distribution from
+    # disjoint seeds, tuned for every form (queueing in a pass depends on the
+    # batch size, so a smaller batch picks other widths)
+    calib_init = True
     for f in forms:
         if path == 1:
             plans[f] = None
             continue
         if f in plans:
             continue
-        plans[f] = autotune.calibrate(form_setup(native, f, meta), kind=1, nsys=1024, seed0=90001, nmin=NMIN,
-                                      nmax=NMAX, nstep_max=NSTEP_MAX, log=log)
+        plans[f] = autotune.calibrate(form_setup(native, f, meta), kind=1, nsys=args.systems, seed0=CALIB_SEED0,
+                                      nmin=NMIN, nmax=NMAX, nstep_max=NSTEP_MAX, log=log, init=calib_init)
+        calib_init = False
     if args.plans_out:
         with open(args.plans_out, "w") as fh:
             json.dump(plans, fh)
@@ -337,8 +343,9 @@ def main():
               "mean_steps": round(float(r["nsteps"].mean()), 1),
               "device_ms_per_step": round(r["dev_ms"], 3),
               "first_pass_ms": round(r["first_ms"], 3) if r["first_ms"] else None,
-              "scheduling": "queue order from the autotuned cost model steps(N)*N^2 (fitted on 1024 systems of "
-                            "seeds 90001..); no step counts of earlier passes of the benchmark batch are used",
+              "scheduling": "queue order from the autotuned cost model steps(N)*N^2, scheduling mode and class widths "
+                            "tuned on %d systems of seeds %d..; no step counts of earlier passes of the benchmark "
+                            "batch are used" % (args.systems, CALIB_SEED0),
               "plan": plans[head]}
     for k, v in res.items():
         config["%s_relax_per_s" % k] = round(v["value"], 2)

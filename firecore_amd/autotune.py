@@ -15,8 +15,10 @@ choice is made with knowledge of the relaxed systems themselves):
   the narrowest enabled width that holds it), timed the same way.
 
 The result is a plain dict (stored by bench.py with its output); apply() sets
-it on the library.  Calibration relaxes the batch a few times, about a second
-of device time for the default 512 systems.
+it on the library.  Calibration relaxes the batch 23 times (about 9 s of
+device time for 4096 systems of the learned form).  The batch should have the
+size of the batch that is then relaxed: how long systems queue behind the
+class launches, and so which widths are best, depends on the batch size.
 """
 import time
 
@@ -55,12 +57,19 @@ def _time_pass(nstep_max, fconv, reps=2):
     return float(np.median(ms))
 
 
+# every set of widths that keeps the 256-thread class (bit c enables width 64*(c+1))
+MASKS = (0xF, 0xE, 0xD, 0xC, 0xB, 0xA, 0x9, 0x8)
+
+
 def calibrate(setup_form, kind=1, nsys=512, seed0=90001, nmin=16, nmax=256, surf="synth:NaCl", nstep_max=10000,
-              fconv=1e-3, log=None):
+              fconv=1e-3, log=None, init=True):
     """Tune the plan for one form.  setup_form() installs the form (precond
-    mode, weights) after the calibration batch is built.  Returns the plan."""
+    mode, weights) after the calibration batch is built.  init=False reuses
+    the batch already loaded (the same calibration batch for several forms;
+    every relaxation restarts from its start positions).  Returns the plan."""
     t0 = time.perf_counter()
-    native.init_workload(kind, nsys, seed0, nmin, nmax, 0.1, surf, 0.1)
+    if init:
+        native.init_workload(kind, nsys, seed0, nmin, nmax, 0.1, surf, 0.1)
     setup_form()
     native.set_cost_model(0, 1000.0, 20.0)
     native.set_sched(1)
@@ -79,7 +88,7 @@ def calibrate(setup_form, kind=1, nsys=512, seed0=90001, nmin=16, nmax=256, surf
     best_sched = int(_pick(trials, "sched1")[5:])
     native.set_sched(best_sched)
     widths = {}
-    for mask in (0xF, 0xB, 0xD, 0x9):  # bit c enables width 64*(c+1); 256 always on
+    for mask in MASKS:
         native.set_class_mask(mask)
         widths[mask] = _time_pass(nstep_max, fconv)
     best_mask = int(_pick(widths, 0xF))
