@@ -160,21 +160,29 @@ def measure(native, sync, steps, warmup, tag, nstep_max=NSTEP_MAX, path=0):
                 status=status, nsteps=nsteps, conv=conv, value=conv / (ms_per_step / 1e3), B=B)
 
 
-def roofline(r, learned, every=PROP_EVERY):
-    """Roofline of the dominant kernel (the size-class launch that ends last).
+def roofline(r, learned, every=PROP_EVERY, plan=None):
+    """Roofline of the relaxation kernel over one pass.
 
-    The persistent kernel keeps every system in LDS and registers; it is bound
-    by FP32 arithmetic (the half-ring pair loop, the B-spline GridFF sample,
-    the bonded terms), so the roof is the chip's FP32 peak (157.3 TFLOP/s;
-    that figure assumes packed FP32, which this build never emits, so 50 % of
-    it is the reachable ceiling for FMA-only code).  achieved = the launch's
-    algorithmic FLOPs (per-unit counts in FLOP_*) over its average duration,
-    measured with HIP events on the launch's stream.  traffic = the launch's
-    HBM-side bytes measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    copy-calibrated; profiles/), and hbm_frac that traffic's share of 8 TB/s
-    over the same duration; grid_gather_bytes are the algorithmic B-spline
-    coefficient bytes (1 KiB per atom-step), most of which the per-atom window
-    cache serves without touching the grid."""
+    A pass launches relax_persistent_kernel once per size class (the same code
+    at a few workgroup widths) on its own stream; the launches overlap and
+    share the device, and persistent workgroups leave only when their class
+    queue is empty, so a class's systems start when the previous launch
+    drains.  The unit is therefore the pass: achieved = the algorithmic FLOPs
+    of all systems of the pass (per-unit counts in FLOP_*) over the pass's
+    device time, measured with HIP events on the stream that forks and joins
+    the class launches.  The kernel keeps every system in LDS and registers
+    and is bound by FP32 arithmetic (the half-ring pair loop, the B-spline
+    GridFF sample, the bonded terms), so the roof is the chip's FP32 peak
+    (157.3 TFLOP/s, a packed-FP32 figure; this build never emits packed FP32,
+    so half of it is the reachable ceiling).  traffic = the HBM-side bytes of
+    all class launches, measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated; profiles/), hbm_frac its share of 8 TB/s over
+    the pass.  grid_gather_bytes: the algorithmic B-spline coefficient bytes
+    (1 KiB per atom and step), most of which the per-atom window cache serves
+    without touching the grid.  `launch` gives the same figures for the class
+    launch that ends last, over its own HIP-event duration on its stream.
+    traffic_plan: the launch plan the counters were measured with (the bytes
+    of a class launch depend on which systems the plan gives it)."""
     B, steps, classes = r["B"], r["nsteps"], r["classes"]
     flops, gbytes = [], []
     for c in classes:
@@ -190,30 +198,38 @@ def roofline(r, learned, every=PROP_EVERY):
                 f += (st // every + 1) * float(FLOP_MLP_ATOM * N)
         flops.append(f)
         gbytes.append(g)
-    dom = int(np.argmax([c["ms"] for c in classes]))
-    dom_ms = float(np.mean([cm[dom] for cm in r["class_ms"]]))
-    tflops = flops[dom] / (dom_ms * 1e-3) / 1e12
     kind = 2 if learned else 0
-    name = "relax_persistent_kernel<%d, false, %d>" % (classes[dom]["block"], kind)
-    traffic = None
-    hbm_frac = None
+    names = ["relax_persistent_kernel<%d, false, %d>" % (c["block"], kind) for c in classes]
+    rec = {}
     tf = os.path.join(ROOT, "profiles", "relax_persistent_traffic.json")
     if os.path.exists(tf):
-        rec = json.load(open(tf)).get(name)
-        if rec:
-            traffic = rec["bytes_per_launch"]
-            hbm_frac = round(traffic / (dom_ms * 1e-3) / (PEAK_HBM_GBS * 1e9), 4)
-    # the class launches of a pass run concurrently and share the device, so the
-    # dominant launch's rate understates the device's: the whole pass as well
-    pass_tflops = float(np.sum(flops)) / (r["dev_ms"] * 1e-3) / 1e12
-    return dict(bound="mfma", achieved=round(tflops, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                frac=round(tflops / PEAK_FP32_TFLOPS, 4), traffic=traffic,
-                pass_flop=float(np.sum(flops)), pass_ms=round(r["dev_ms"], 3), pass_achieved=round(pass_tflops, 3),
-                pass_frac=round(pass_tflops / PEAK_FP32_TFLOPS, 4),
+        rec = json.load(open(tf))
+    pass_ms = r["dev_ms"]
+    pass_flop = float(np.sum(flops))
+    pass_tflops = pass_flop / (pass_ms * 1e-3) / 1e12
+    live = [k for k, c in enumerate(classes) if c["nsys"] > 0]
+    traffic = float(sum(rec[names[k]]["bytes_per_launch"] for k in live)) if all(names[k] in rec for k in live) \
+        else None
+    dom = int(np.argmax([c["ms"] for c in classes]))
+    dom_ms = float(np.mean([cm[dom] for cm in r["class_ms"]]))
+    dom_traffic = rec[names[dom]]["bytes_per_launch"] if names[dom] in rec else None
+    launch = dict(kernel=names[dom], ms=round(dom_ms, 3), systems=classes[dom]["nsys"],
+                  algorithmic_flop=flops[dom], achieved=round(flops[dom] / (dom_ms * 1e-3) / 1e12, 3),
+                  frac=round(flops[dom] / (dom_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4), traffic=dom_traffic)
+    return dict(bound="mfma", achieved=round(pass_tflops, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                frac=round(pass_tflops / PEAK_FP32_TFLOPS, 4), traffic=traffic,
                 pipe="FP32 vector ALU (the step loop; the learned form's in-kernel network adds BF16 MFMA "
-                     "work of under 1 % of the launch's cycles); peak = the FP32 figure",
-                kernel=name, kernel_ms=round(dom_ms, 3), kernel_systems=classes[dom]["nsys"],
-                algorithmic_flop=flops[dom], hbm_frac=hbm_frac, grid_gather_bytes=gbytes[dom])
+                     "work of under 1 % of the kernel's cycles); peak = the FP32 figure",
+                kernel="relax_persistent_kernel, the %d size-class launches of one pass (%s)"
+                       % (len(live), ", ".join(names[k] for k in live)),
+                kernel_ms=round(pass_ms, 3), kernel_systems=int(sum(classes[k]["nsys"] for k in live)),
+                algorithmic_flop=pass_flop,
+                hbm_frac=round(traffic / (pass_ms * 1e-3) / (PEAK_HBM_GBS * 1e9), 4) if traffic else None,
+                grid_gather_bytes=float(np.sum(gbytes)), launch=launch,
+                traffic_plan=dict(rec.get("_profiled_plans", {}).get("learned" if learned else "plain", {}),
+                                  same_as_this_pass=bool(plan) and all(
+                                      rec.get("_profiled_plans", {}).get("learned" if learned else "plain", {}).get(k)
+                                      == plan.get(k) for k in ("sched", "class_mask"))))
 
 
 def load_trained(native):
@@ -360,8 +376,8 @@ def main():
             int(meta.get("every", PROP_EVERY)), meta["gstar"], meta["grigid"], meta["pairs"], meta["seeds"])
     if path == 1:
         config["path"] = "split kernels (profiling mode)"
-    rl = roofline(r, head == "learned", int(meta.get("every", PROP_EVERY)) if meta else PROP_EVERY) if path == 0 \
-        else None
+    rl = roofline(r, head == "learned", int(meta.get("every", PROP_EVERY)) if meta else PROP_EVERY,
+                  plans.get(head)) if path == 0 else None
     if not args.no_screening and meta:
         config["screening"] = screening_leg(native, sync, autotune, meta, args.screening_systems, 1)
     native.set_precond(0)
