@@ -215,6 +215,7 @@ int World::relax_learned(int nstep_max, double Fconv, bool restore_start) {
     // exact forces at the start (first network evaluation); resets status, steps and FIRE state
     if (!eval_forces()) return -1;
     fc_launch_zero_f4(stream, d_avel.p, ntotV);
+    fc_launch_zero_f4(stream, d_stall.p, nSys);  // fallback flags of this relaxation only
     fcd::BatchDev B = batch_dev();
     fcd::RunParams P = run_params(Fconv, nstep_max);
     P.chunk = std::max(pc_every, 1);

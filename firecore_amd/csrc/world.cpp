@@ -21,6 +21,7 @@ extern "C" hipError_t fc_launch_relax_persistent(int block, int hb, int ngroups,
 extern "C" int fc_hslot_elems(int nmax);
 extern "C" hipError_t fc_noalloc_scratch(void** ptr);
 extern "C" hipError_t fc_launch_zero_int(hipStream_t st, int* a, int n);
+extern "C" hipError_t fc_launch_zero_f4(hipStream_t st, float4* a, int n);
 extern "C" int fc_lds_bytes(int nmax, int mmax, int wmax, int block, int learned, int h2lds);
 extern "C" hipError_t fc_launch_split_step(hipStream_t st, const fcd::BatchDev* B, const fcd::RunParams* P,
                                            const int* node_sys, int ntot_nodes, int maxN, int do_move, int step);
@@ -764,6 +765,7 @@ int World::relax_persistent(int nstep_max, double Fconv, bool restore_start) {
         hip_ok(hipMemcpyAsync(d_atoms.p, d_atoms0.p, sizeof(float4) * ntotV, hipMemcpyDeviceToDevice, stream),
                "restore");
     if (record_times) d_systime.alloc((size_t)2 * nSys);
+    fc_launch_zero_f4(stream, d_stall.p, nSys);  // fallback flags of this relaxation only
     fcd::BatchDev B = batch_dev();
     fcd::RunParams P = run_params(Fconv, nstep_max);
     std::vector<int> zeros(8, 0);
@@ -875,6 +877,7 @@ int World::relax_noalloc(int nstep_max, double Fconv, bool restore_start) {
     B.systime = nullptr;
     B.gridcache = reinterpret_cast<float4*>(base + off_cache);
     B.gridcache_stride = ntotA;
+    fc_launch_zero_f4(stream, d_stall.p, nSys);  // fallback flags of this relaxation only
     fcd::RunParams P = run_params(Fconv, nstep_max);
     hipEvent_t fork = noalloc_events[0];
     hipEventRecord(fork, stream);
@@ -1031,6 +1034,7 @@ int World::relax_split(int nstep_max, double Fconv, bool restore_start) {
     fcd::BatchDev B = batch_dev();
     fcd::RunParams P = run_params(Fconv, nstep_max);
     hipMemsetAsync(d_avel.p, 0, sizeof(float4) * ntotV, stream);
+    fc_launch_zero_f4(stream, d_stall.p, nSys);  // no learned form here: no fallback flags
     fc_launch_reset(stream, &B, &P);
     EventSet events;
     hipEvent_t e0 = events.make(), e1 = events.make();

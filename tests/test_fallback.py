@@ -64,6 +64,10 @@ def test_fallback_systems_reach_oracle_minima(orc, inkernel):
         assert (np.array(B.status) == native.ST_CONVERGED).all(), np.array(B.status)
         assert fb.sum() >= 5, fb.sum()  # most systems plateau for two windows at some point
         _check_relaxed_against_oracle(orc, B, 1, seed0, nmin, nmax, range(n))
+        # the flags describe the last relaxation only: a constant-gain one clears them
+        native.set_precond(1, *BEST_CONST)
+        native.relax(20000, 1e-3, 0, True, True)
+        assert not native.proposal_fallback().any()
     finally:
         native.set_proposal_stall(0)
         native.set_learned_inkernel(old)
