@@ -373,8 +373,11 @@ int make_back_neighs(int natoms, int nnode, const int* neighs, int* bkneighs);
  * its template (the reference's MolWorld_sp3::init with bMMFF: MMFFBuilder
  * typing, bonds, toMMFFsp3_loc, MolWorld_sp3.h:309-357, 1143-1440), with the
  * type tables fElem/fAtom/fBond (ElementTypes.dat, AtomTypes.dat,
- * BondTypes.dat).  Outputs in the MMFFsp3_loc layout; lvec9 and *periodic
- * report an lvs line.  Returns natoms, -1 on error, -2 above maxn atoms. */
+ * BondTypes.dat).  Outputs in the MMFFsp3_loc layout, sized for maxn atoms:
+ * apos 2*maxn x 3 (atoms, then one pi-vector per node), REQs maxn x 4, neighs
+ * maxn x 4, apars/bLs/bKs/Ksp/Kpp maxn x 4, elem maxn; lvec9 and *periodic
+ * report an lvs line.  Returns natoms, -1 on error, -2 above maxn atoms
+ * (nothing written). */
 int build_xyz_arrays(const char* xyz, const char* fElem, const char* fAtom, const char* fBond, int maxn,
                      int* nnode_out, double* apos, double* REQs, int* neighs, double* apars, double* bLs, double* bKs,
                      double* Ksp, double* Kpp, int* elem, double* lvec9, int* periodic);
