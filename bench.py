@@ -101,29 +101,42 @@ def host_info():
     return dict(model=model, nproc=os.cpu_count() or 1, affinity=affinity)
 
 
-def cpu_baseline(native, grid, nsample, nthreads, reps=5, nsys=NSYS):
-    """FP64 oracle of the reference CPU path (plain FIRE, one system per
+def cpu_baseline(native, grid, nsample, nthreads, reps=5, nsys=NSYS, reps32=3):
+    """The oracle of the reference CPU path (plain FIRE, one system per
     thread, dynamic schedule) on an evenly spaced sample of the workload:
-    one warm-up run, then the median of `reps` timed runs."""
+    FP64 (the reported value), one warm-up run, then the median of `reps`
+    timed runs; and the same restatement built in single precision (every
+    array and operation FP32), median of `reps32` after a warm-up."""
     import oracle
     idx = np.linspace(0, nsys - 1, nsample).astype(int)
     systems = [native.gen_workload_system(1, int(i), SEED0, NMIN, NMAX, 0.1, grid) for i in idx]
     fire = tuple(FIRE) + (FLIMIT,)
-    rates, convs = [], []
-    for r in range(reps + 1):
-        t = time.perf_counter()
-        steps, E, Fm, _ = oracle.relax_batch(systems, fire, FCONV, NSTEP_MAX, 0, grid=grid, nthreads=nthreads)
-        dt = time.perf_counter() - t
-        conv = int(np.sum(np.isfinite(Fm) & (Fm < FCONV)))
-        if r > 0:
-            rates.append(conv / dt)
-            convs.append(conv)
-        log("cpu baseline run %d: %d converged in %.2f s" % (r, conv, dt))
+
+    def timed(precision, nrep):
+        rates, convs = [], []
+        for r in range(nrep + 1):
+            t = time.perf_counter()
+            steps, E, Fm, _ = oracle.relax_batch(systems, fire, FCONV, NSTEP_MAX, 0, grid=grid, nthreads=nthreads,
+                                                 precision=precision)
+            dt = time.perf_counter() - t
+            conv = int(np.sum(np.isfinite(Fm) & (Fm < FCONV)))
+            if r > 0:
+                rates.append(conv / dt)
+                convs.append(conv)
+            log("cpu baseline %s run %d: %d converged in %.2f s" % (precision, r, conv, dt))
+        return rates, convs
+
+    rates, convs = timed("f64", reps)
+    rates32, convs32 = timed("f32", reps32)
     h = host_info()
     return dict(value=float(np.median(rates)), unit="relaxations/s", cores=nthreads, kind="port",
                 sample="%d of the %d systems (evenly spaced indices), FP64 oracle of the reference CPU path, "
                        "plain FIRE, same limits; %d converged; median of %d runs after 1 warm-up (%.2f..%.2f/s)"
                        % (nsample, nsys, convs[0], reps, min(rates), max(rates)),
+                fp32=dict(value=float(np.median(rates32)), converged=convs32[0],
+                          note="the same restatement in single precision, median of %d runs after 1 warm-up "
+                               "(%.2f..%.2f/s); scalar code, so FP32 gains little over FP64"
+                               % (reps32, min(rates32), max(rates32))),
                 host=h["model"], host_nproc=h["nproc"], host_affinity=h["affinity"],
                 threads_note="one system per thread on %d threads: this box's CPU share per GPU" % nthreads)
 

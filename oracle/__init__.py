@@ -359,9 +359,12 @@ def req2plq(REQ, alpha):
     return out
 
 
-def relax_batch(systems, fire, Fconv, nmax, conv_mode=0, sw=(1, 1, 1, 1, 1), grid=None, nthreads=1):
-    """Relax a list of systems on `nthreads` host threads (FP64).  Returns
-    (steps, E, Fmax, positions list)."""
+def relax_batch(systems, fire, Fconv, nmax, conv_mode=0, sw=(1, 1, 1, 1, 1), grid=None, nthreads=1,
+                precision="f64"):
+    """Relax a list of systems on `nthreads` host threads: FP64 (the oracle),
+    or precision="f32" for the same restatement with every array and
+    operation in single precision (the FP32 CPU baseline, not a parity
+    reference).  Returns (steps, E, Fmax, positions list)."""
     lib = ctypes.CDLL(ORACLE_SO)
     ns = len(systems)
     natoms = np.array([m["natoms"] for m in systems], np.int32)
@@ -376,10 +379,11 @@ def relax_batch(systems, fire, Fconv, nmax, conv_mode=0, sw=(1, 1, 1, 1, 1), gri
     E = np.zeros(ns)
     Fm = np.zeros(ns)
     g = Backend._grid_args(grid)
-    lib.orc_relax_batch(ctypes.c_int(ns), _p(natoms, _ip), _p(nnode, _ip), _p(apos), _p(REQs), _p(neighs, _ip),
-                        _p(apars), _p(bLs), _p(bKs), _p(Ksp), _p(Kpp), _p(sw, _ip), *g, _p(fire),
-                        ctypes.c_double(Fconv), ctypes.c_int(nmax), ctypes.c_int(conv_mode), ctypes.c_int(nthreads),
-                        _p(steps, _ip), _p(E), _p(Fm))
+    fn = lib.orc_relax_batch_f32 if precision == "f32" else lib.orc_relax_batch
+    fn(ctypes.c_int(ns), _p(natoms, _ip), _p(nnode, _ip), _p(apos), _p(REQs), _p(neighs, _ip),
+       _p(apars), _p(bLs), _p(bKs), _p(Ksp), _p(Kpp), _p(sw, _ip), *g, _p(fire),
+       ctypes.c_double(Fconv), ctypes.c_int(nmax), ctypes.c_int(conv_mode), ctypes.c_int(nthreads),
+       _p(steps, _ip), _p(E), _p(Fm))
     out = []
     off = 0
     for m in systems:
