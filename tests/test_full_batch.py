@@ -1,0 +1,90 @@
+"""The headline workload at full size: all 4096 systems (N ~ U[16,256], seeds
+10001.., NaCl GridFF), relaxed on the device in each benchmarked form with the
+benchmark's settings (set_opt defaults, per-vec clamp 10 eV/A, at most 10000
+steps), and every converged system confirmed by the FP64 oracle at the
+device's final geometry: max|F| < 1.01e-3 eV/A over all vecs (atoms and
+pi-vectors) and |E_dev - E_oracle| < 1e-4 eV.  One oracle force evaluation per
+system, so the whole batch is checked, not a subset.
+
+The learned form uses the trained weights with their stored period and gain
+ceilings, the constant form the best constant gains (star 2, molecule 127);
+both converge every system.  Plain FIRE leaves some systems at the step limit
+(status ST_MAXSTEP, checked to have used exactly the limit); every system it
+converges is checked.
+"""
+import json
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from firecore_amd import native
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NSYS, SEED0, NMIN, NMAX = 4096, 10001, 16, 256
+FIRE_MULTI = (0.1, 0.02, 0.2, 1.1, 0.5, 0.8, 5)
+NSTEP_MAX = 10000
+
+
+@pytest.fixture(scope="module")
+def batch():
+    native.set_opt(*FIRE_MULTI)
+    native.set_force_limit(10.0)
+    native.set_conv_mode(0)
+    print("case: full headline batch, %d systems, seeds %d.." % (NSYS, SEED0), flush=True)
+    native.init_workload(1, NSYS, SEED0, NMIN, NMAX, 0.1, "synth:NaCl", 0.1)
+    grid = native.grid_info()
+    # the oracle's inputs: every system generated again on the host in double
+    # precision (the generator is thread-safe and releases the GIL)
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        systems = list(ex.map(lambda s: native.gen_workload_system(1, s, SEED0, NMIN, NMAX, 0.1, grid), range(NSYS)))
+    yield grid, systems
+    native.set_precond(0)
+    native.set_force_limit(0.0)
+    native.set_opt()
+
+
+def _form(form):
+    if form == "plain":
+        native.set_precond(0)
+    elif form == "constant":
+        native.set_precond(1, 2.0, 127.0)
+    else:
+        meta = json.load(open(os.path.join(ROOT, "firecore_amd", "proposal_weights.json")))
+        native.set_proposal_weights(np.load(os.path.join(ROOT, "firecore_amd", "proposal_weights.npy")))
+        native.set_proposal_every(int(meta["every"]))
+        native.set_precond(2, meta["gstar"], meta["grigid"])
+
+
+@pytest.mark.parametrize("form", ["learned", "constant", "plain"])
+def test_full_batch_minima_confirmed_by_oracle(orc, batch, form):
+    grid, systems = batch
+    _form(form)
+    print("case: full batch,", form, flush=True)
+    native.relax(NSTEP_MAX, 1e-3, 0, True, True)
+    B = native.Batch()
+    st = np.array(B.status)
+    steps = np.array(B.nsteps)
+    conv = np.where(st == native.ST_CONVERGED)[0]
+    others = np.where(st != native.ST_CONVERGED)[0]
+    if form != "plain":
+        assert len(others) == 0, (form, others[:10], st[others[:10]])
+    else:
+        assert len(conv) > 0.95 * NSYS, len(conv)
+        assert (st[others] == native.ST_MAXSTEP).all(), np.unique(st[others])
+        assert (steps[others] == NSTEP_MAX).all()
+    worst_f = worst_e = 0.0
+    for s in conv:
+        s = int(s)
+        m = dict(systems[s], apos=B.positions(s))
+        f, _, _, E = orc.eval(m, grid=grid)
+        fm = float(np.sqrt((f ** 2).sum(1)).max())
+        de = abs(float(B.E[s]) - E)
+        assert fm < 1.01e-3, (form, s, m["natoms"], fm)
+        assert de < 1e-4, (form, s, m["natoms"], B.E[s], E)
+        worst_f, worst_e = max(worst_f, fm), max(worst_e, de)
+    print("%s: %d converged (mean %.0f steps), worst oracle max|F| %.3e eV/A, worst |dE| %.2e eV"
+          % (form, len(conv), steps.mean(), worst_f, worst_e), flush=True)
