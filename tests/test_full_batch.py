@@ -10,7 +10,8 @@ The learned form uses the trained weights with their stored period and gain
 ceilings, the constant form the best constant gains (star 2, molecule 127);
 both converge every system.  Plain FIRE leaves some systems at the step limit
 (status ST_MAXSTEP, checked to have used exactly the limit); every system it
-converges is checked.
+converges is checked.  The screening workload (65,536 molecule pairs) is
+relaxed whole as well, with a strided sample of its systems confirmed.
 """
 import json
 import os
@@ -88,3 +89,48 @@ def test_full_batch_minima_confirmed_by_oracle(orc, batch, form):
         worst_f, worst_e = max(worst_f, fm), max(worst_e, de)
     print("%s: %d converged (mean %.0f steps), worst oracle max|F| %.3e eV/A, worst |dE| %.2e eV"
           % (form, len(conv), steps.mean(), worst_f, worst_e), flush=True)
+
+
+def test_full_screening_batch_minima_confirmed_by_oracle(orc):
+    """The screening workload at full size: 65,536 molecule pairs (8-64 atoms
+    each, seeds 20001..) relaxed by the learned form (trained weights, at most
+    3000 steps) in one batch.  At least 99 % converge, the rest stop at the
+    step limit; every 16th system that converged (about 4000, spread over the
+    whole batch) is confirmed by the FP64 oracle (max|F| < 1.01e-3 eV/A over
+    all vecs, |dE| < 1e-4 eV), intermolecular pairs included."""
+    nsys, seed0, nmax_steps = 65536, 20001, 3000
+    native.set_opt(*FIRE_MULTI)
+    native.set_force_limit(10.0)
+    native.set_conv_mode(0)
+    try:
+        print("case: full screening batch, %d molecule pairs, seeds %d.." % (nsys, seed0), flush=True)
+        native.init_workload(2, nsys, seed0, 8, 64, 0.1, "synth:NaCl", 0.1)
+        grid = native.grid_info()
+        _form("learned")
+        native.relax(nmax_steps, 1e-3, 0, True, True)
+        B = native.Batch()
+        st = np.array(B.status)
+        steps = np.array(B.nsteps)
+        conv = np.where(st == native.ST_CONVERGED)[0]
+        others = np.where(st != native.ST_CONVERGED)[0]
+        assert len(conv) >= 0.99 * nsys, len(conv)
+        assert (st[others] == native.ST_MAXSTEP).all() and (steps[others] == nmax_steps).all()
+        check = conv[::16]
+        with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+            systems = list(ex.map(lambda s: native.gen_workload_system(2, int(s), seed0, 8, 64, 0.1, grid), check))
+        worst_f = worst_e = 0.0
+        for s, m0 in zip(check, systems):
+            s = int(s)
+            m = dict(m0, apos=B.positions(s))
+            f, _, _, E = orc.eval(m, grid=grid)
+            fm = float(np.sqrt((f ** 2).sum(1)).max())
+            de = abs(float(B.E[s]) - E)
+            assert fm < 1.01e-3, (s, m["natoms"], fm)
+            assert de < 1e-4, (s, m["natoms"], B.E[s], E)
+            worst_f, worst_e = max(worst_f, fm), max(worst_e, de)
+        print("screening: %d of %d converged (mean %.0f steps), %d checked, worst oracle max|F| %.3e eV/A, "
+              "worst |dE| %.2e eV" % (len(conv), nsys, steps.mean(), len(check), worst_f, worst_e), flush=True)
+    finally:
+        native.set_precond(0)
+        native.set_force_limit(0.0)
+        native.set_opt()
