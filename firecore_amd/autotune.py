@@ -14,6 +14,10 @@ choice is made with knowledge of the relaxed systems themselves):
 * the set of workgroup widths (64, 128, 192, 256 threads; a system goes to
   the narrowest enabled width that holds it), timed the same way.
 
+An alternative replaces the default setting only when it is at least MARGIN
+faster; among alternatives within MARGIN of the fastest, the width set with
+fewer class launches is taken.
+
 The result is a plain dict (stored by bench.py with its output); apply() sets
 it on the library.  Calibration relaxes the batch 23 times (about 9 s of
 device time for 4096 systems of the learned form).  The batch should have the
@@ -44,9 +48,15 @@ def fit_steps(N, steps):
 MARGIN = 0.015
 
 
-def _pick(times, default):
+def _pick(times, default, rank=None):
+    """The default unless an alternative is at least MARGIN faster.  Among the
+    alternatives within MARGIN of the fastest, the lowest rank(key) wins (ties:
+    the faster), so that near-equal settings do not flip from run to run."""
     best = min(times, key=times.get)
-    return best if times[best] < times[default] * (1.0 - MARGIN) else default
+    if not times[best] < times[default] * (1.0 - MARGIN):
+        return default
+    near = [k for k in times if k != default and times[k] <= times[best] * (1.0 + MARGIN)]
+    return min(near, key=lambda k: ((rank(k) if rank else 0), times[k]))
 
 
 def _time_pass(nstep_max, fconv, reps=2):
@@ -91,7 +101,9 @@ def calibrate(setup_form, kind=1, nsys=512, seed0=90001, nmin=16, nmax=256, surf
     for mask in MASKS:
         native.set_class_mask(mask)
         widths[mask] = _time_pass(nstep_max, fconv)
-    best_mask = int(_pick(widths, 0xF))
+    # near-ties go to the plan with fewer class launches (fewer points where a
+    # class waits for the previous launch to drain)
+    best_mask = int(_pick(widths, 0xF, rank=lambda m: bin(m).count("1")))
     native.set_class_mask(best_mask)
     plan = dict(cost_c0=round(c0, 3), cost_c1=round(c1, 4), sched=best_sched, class_mask=best_mask,
                 widths=[64 * (c + 1) for c in range(4) if (best_mask >> c) & 1],
