@@ -894,7 +894,7 @@ int relax(int nstepMax, double Fconv, int path, bool restore_start, bool bDownlo
     World& W = world();
     if (!W.dev_ready || W.nSys == 0) return -1;
     int n = (path == 1) ? W.relax_split(nstepMax, Fconv, restore_start) : W.relax_persistent(nstepMax, Fconv, restore_start);
-    if (n >= 0 && bDownload) W.download_all(true, true);
+    if (n >= 0 && bDownload && !W.download_all(true, true)) return -1;
     return n;
 }
 
@@ -1036,6 +1036,15 @@ int set_class_mask(int mask) {
     W.class_mask = mask | 0x8;
     if (W.dev_ready && W.nSys > 0) W.plan_classes();
     return old;
+}
+
+// Waits for all work on the device and returns 0, or the HIP error code of a
+// failure that earlier calls have not reported (message on stderr).
+int device_status() {
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) std::fprintf(stderr, "[libMMFFmulti_lib] device status: %s\n", hipGetErrorString(e));
+    return (int)e;
 }
 
 int set_record_times(int on) {

@@ -287,7 +287,7 @@ int World::relax_learned(int nstep_max, double Fconv, bool restore_start) {
     if (n_unclassed > 0 && !relax_unclassed(nstep_max, Fconv)) return -1;
     d_nsteps.download(nsteps, stream);
     d_status.download(status, stream);
-    hip_ok(hipStreamSynchronize(stream), "learned relax results");
+    if (!hip_ok(hipStreamSynchronize(stream), "learned relax results")) return -1;
     int mx = 0;
     for (int s = 0; s < nSys; s++) mx = std::max(mx, nsteps[s]);
     return mx;

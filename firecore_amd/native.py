@@ -533,6 +533,14 @@ def set_class_mask(mask):
     return r
 
 
+def device_status():
+    """0, or the HIP error code of a device failure not reported by an earlier call."""
+    L = lib()
+    L.device_status.argtypes = []
+    L.device_status.restype = c_int
+    return L.device_status()
+
+
 def set_record_times(on):
     L = lib()
     L.set_record_times.argtypes = [c_int]

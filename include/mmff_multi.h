@@ -290,6 +290,12 @@ int set_cost_model(int feedback, double c0, double c1);
  * goes to the narrowest enabled width that holds it.  Returns the previous
  * mask, -1 if mask is outside 0..15. */
 int set_class_mask(int mask);
+/* Waits for all work on the device; 0, or the HIP error code of a failure no
+ * earlier call has reported (message on stderr).  No counterpart in the
+ * reference (its OpenCL queue reports errors per call); a test harness calls it
+ * after each device test so that an asynchronous fault is charged to the test
+ * that caused it. */
+int device_status(void);
 /* Record each system's start and end (device wall clock) in later persistent
  * relaxations (1) or not (0, default).  Returns the previous setting. */
 int set_record_times(int on);

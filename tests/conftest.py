@@ -12,6 +12,17 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
 
 
+@pytest.fixture(autouse=True)
+def _device_clean_after_gpu_test(request):
+    """After every device test, wait for the device and fail that test if it
+    left an unreported device error (asynchronous faults surface at a later
+    call otherwise, and would be charged to whichever test makes it)."""
+    yield
+    if request.node.get_closest_marker("gpu") is not None:
+        from firecore_amd import native
+        assert native.device_status() == 0, "device error after " + request.node.nodeid
+
+
 @pytest.fixture(scope="session")
 def orc():
     import oracle
