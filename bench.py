@@ -24,7 +24,8 @@ scheduling mode, workgroup widths) is tuned by firecore_amd/autotune.py on
 4096 systems of seeds 90001.. (disjoint from the benchmark batch); the timed
 passes never use step counts of earlier passes over the benchmark batch.
 A second leg relaxes the screening workload: 65,536 molecule pairs (8-64
-atoms each, seeds 20001..), learned form, at most 3000 steps.
+atoms each, seeds 20001..), learned form, at most 3000 steps, with its launch
+plan tuned on a batch of the same size from seeds 190001...
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-screening]
 """
@@ -64,6 +65,7 @@ FLOP_PC_NODE, FLOP_PC_ATOM = 200, 60
 FLOP_MLP_ATOM = 2 * (16 * 64 + 2 * 128 * 64 + 2 * 64)
 PROP_EVERY = 8
 SCREEN_NSYS, SCREEN_SEED0, SCREEN_STEPS = 65536, 20001, 3000
+SCREEN_CALIB_SEED0 = 190001  # calibration batch of the screening plan (disjoint from 20001.. and 10001..)
 
 
 def log(*a):
@@ -269,8 +271,10 @@ def screening_leg(native, sync, autotune, meta, nsys, steps):
     atoms (random relative orientation, 4-8 A apart) over the same GridFF,
     learned form, at most 3000 steps."""
     setup = form_setup(native, "learned", meta)
-    plan = autotune.calibrate(setup, kind=2, nsys=2048, seed0=190001, nmin=8, nmax=64, nstep_max=SCREEN_STEPS,
-                              log=log)
+    # calibration batch of the same size from disjoint seeds (queueing, and so
+    # the best widths, depend on the batch size)
+    plan = autotune.calibrate(setup, kind=2, nsys=nsys, seed0=SCREEN_CALIB_SEED0, nmin=8, nmax=64,
+                              nstep_max=SCREEN_STEPS, log=log)
     t0 = time.perf_counter()
     native.init_workload(2, nsys, SCREEN_SEED0, 8, 64, 0.1, "synth:NaCl", 0.1)
     log("screening init %.1f s" % (time.perf_counter() - t0))
