@@ -91,6 +91,36 @@ def test_full_batch_minima_confirmed_by_oracle(orc, batch, form):
           % (form, len(conv), steps.mean(), worst_f, worst_e), flush=True)
 
 
+def test_full_batch_split_path_confirmed_by_oracle(orc, batch):
+    """The same 4096 systems relaxed by the split kernels (one launch per phase
+    and step, the reference's decomposition; plain FIRE): every converged
+    system confirmed by the FP64 oracle (max|F| < 1.01e-3 eV/A over all vecs,
+    |dE| < 1e-4 eV), the rest at the step limit."""
+    grid, systems = batch
+    native.set_precond(0)
+    print("case: full batch, split kernels, plain", flush=True)
+    native.relax(NSTEP_MAX, 1e-3, 1, True, True)
+    B = native.Batch()
+    st = np.array(B.status)
+    steps = np.array(B.nsteps)
+    conv = np.where(st == native.ST_CONVERGED)[0]
+    others = np.where(st != native.ST_CONVERGED)[0]
+    assert len(conv) > 0.95 * NSYS, len(conv)
+    assert (st[others] == native.ST_MAXSTEP).all() and (steps[others] == NSTEP_MAX).all()
+    worst_f = worst_e = 0.0
+    for s in conv:
+        s = int(s)
+        m = dict(systems[s], apos=B.positions(s))
+        f, _, _, E = orc.eval(m, grid=grid)
+        fm = float(np.sqrt((f ** 2).sum(1)).max())
+        de = abs(float(B.E[s]) - E)
+        assert fm < 1.01e-3, (s, m["natoms"], fm)
+        assert de < 1e-4, (s, m["natoms"], B.E[s], E)
+        worst_f, worst_e = max(worst_f, fm), max(worst_e, de)
+    print("split: %d converged (mean %.0f steps), worst oracle max|F| %.3e eV/A, worst |dE| %.2e eV"
+          % (len(conv), steps.mean(), worst_f, worst_e), flush=True)
+
+
 def test_full_screening_batch_minima_confirmed_by_oracle(orc):
     """The screening workload at full size: 65,536 molecule pairs (8-64 atoms
     each, seeds 20001..) relaxed by the learned form (trained weights, at most
